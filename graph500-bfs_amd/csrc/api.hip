@@ -1,0 +1,353 @@
+// api.hip -- the extern "C" entry points declared in include/g500bfs.h.
+//
+// Each entry point sets the handle's device, runs the C++ implementation and
+// turns exceptions into status codes plus a per-thread message.
+#include <cstdarg>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+#include "mrg.hpp"
+
+namespace g500 {
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+}  // namespace g500
+
+namespace {
+
+thread_local std::string t_last_error;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return G500_OK;
+  } catch (const g500::Error& e) {
+    t_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    t_last_error = "host allocation failed";
+    return G500_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    t_last_error = e.what();
+    return G500_ERR_STATE;
+  }
+}
+
+void use_device(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    g500::fail(G500_ERR_NODEV, "no HIP device available");
+  if (device < 0 || device >= count)
+    g500::fail(G500_ERR_ARG, "device %d outside [0, %d)", device, count);
+  G500_CHECK_HIP(hipSetDevice(device));
+}
+
+g500_graph* new_graph(int device) {
+  use_device(device);
+  auto* g = new g500_graph;
+  g->device = device;
+  if (hipStreamCreate(&g->stream) != hipSuccess) {
+    delete g;
+    g500::fail(G500_ERR_HIP, "hipStreamCreate failed");
+  }
+  return g;
+}
+
+void need_graph(const g500_graph* g, bool constructed) {
+  if (!g) g500::fail(G500_ERR_ARG, "null graph");
+  if (constructed && !g->constructed)
+    g500::fail(G500_ERR_STATE, "graph not constructed (call g500_graph_construct)");
+}
+
+bool has_edge_host(g500_graph* g, int64_t v) {
+  if (v < 0 || v >= g->n) return false;
+  g500::ensure_host_orig2new(g);
+  if ((int64_t)g->h_orig2new[(size_t)v] < g->n_c) return true;
+  return g->corebfs && !g->h_tree_parent.empty() && g->h_tree_parent[(size_t)v] >= 0;
+}
+
+}  // namespace
+
+g500_graph::~g500_graph() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+int g500_graph_generate(int device, int scale, int edgefactor, uint64_t seed1,
+                        uint64_t seed2, g500_graph** out) {
+  return guard([&] {
+    if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    g500_graph* g = new_graph(device);
+    try {
+      g500::generate_edges(g, scale, edgefactor, seed1, seed2);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
+                          int64_t nedges, g500_graph** out) {
+  return guard([&] {
+    if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (nedges < 0 || (nedges > 0 && (!v0 || !v1)))
+      g500::fail(G500_ERR_ARG, "bad edge arrays");
+    std::vector<uint2> packed((size_t)nedges);
+    uint64_t maxid = 0;
+    for (int64_t i = 0; i < nedges; ++i) {
+      if (v0[i] < 0 || v1[i] < 0 || v0[i] > 0xFFFFFFFFll || v1[i] > 0xFFFFFFFFll)
+        g500::fail(G500_ERR_ARG, "edge %lld has an endpoint outside [0, 2^32)",
+                   (long long)i);
+      packed[(size_t)i] = make_uint2((uint32_t)v0[i], (uint32_t)v1[i]);
+      maxid |= (uint64_t)(v0[i] | v1[i]);
+    }
+    g500_graph* g = new_graph(device);
+    try {
+      g->nedges = nedges;
+      g->id_bits = maxid ? 64 - __builtin_clzll(maxid) : 0;
+      g->edges.alloc((size_t)nedges);
+      if (nedges)
+        G500_CHECK_HIP(hipMemcpy(g->edges.get(), packed.data(), nedges * sizeof(uint2),
+                                 hipMemcpyHostToDevice));
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int g500_graph_construct(g500_graph* g, int corebfs) {
+  return guard([&] {
+    need_graph(g, false);
+    use_device(g->device);
+    if (g->constructed) g500::fail(G500_ERR_STATE, "graph already constructed");
+    if (corebfs) {
+      g500::corebfs_prune(g);
+    } else {
+      g500::construct_graph(g, 0);
+    }
+  });
+}
+
+int g500_graph_get_info(const g500_graph* g, g500_graph_info* info) {
+  return guard([&] {
+    need_graph(g, false);
+    if (!info) g500::fail(G500_ERR_ARG, "null info");
+    std::memset(info, 0, sizeof(*info));
+    info->log_n = g->log_n;
+    info->corebfs = g->corebfs;
+    info->n = g->n;
+    info->n_c = g->n_c;
+    info->m = g->m;
+    info->m_sym = g->m_sym;
+    info->nedges_input = g->nedges;
+    info->max_used_vertex = g->max_used;
+    info->max_degree = g->max_deg;
+    info->core_vertices = g->core_vertices;
+    info->tree_vertices = g->tree_vertices;
+    info->device_bytes = g->device_bytes();
+  });
+}
+
+int g500_graph_has_edge(g500_graph* g, int64_t v, int* out) {
+  return guard([&] {
+    need_graph(g, true);
+    if (!out) g500::fail(G500_ERR_ARG, "null output");
+    use_device(g->device);
+    *out = has_edge_host(g, v) ? 1 : 0;
+  });
+}
+
+int g500_graph_copy_edges(const g500_graph* g, int64_t start, int64_t count,
+                          int64_t* v0, int64_t* v1) {
+  return guard([&] {
+    need_graph(g, false);
+    if (start < 0 || count < 0 || start + count > g->nedges)
+      g500::fail(G500_ERR_ARG, "edge range outside [0, %lld)", (long long)g->nedges);
+    if (count && (!v0 || !v1)) g500::fail(G500_ERR_ARG, "null output arrays");
+    use_device(g->device);
+    std::vector<uint2> tmp((size_t)count);
+    if (count)
+      G500_CHECK_HIP(hipMemcpy(tmp.data(), g->edges.get() + start, count * sizeof(uint2),
+                               hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < count; ++i) {
+      v0[i] = tmp[(size_t)i].x;
+      v1[i] = tmp[(size_t)i].y;
+    }
+  });
+}
+
+int g500_graph_copy_csr(const g500_graph* g, int64_t* row_off, uint32_t* col,
+                        int64_t* deg_new, uint32_t* new2orig, uint32_t* orig2new) {
+  return guard([&] {
+    need_graph(g, true);
+    use_device(g->device);
+    if (row_off) {
+      G500_CHECK_HIP(hipMemcpy(row_off, g->row_off.get(), (g->n_c + 1) * sizeof(int64_t),
+                               hipMemcpyDeviceToHost));
+      for (int64_t v = g->n_c + 1; v <= g->n; ++v) row_off[v] = g->m;
+    }
+    if (col && g->m)
+      G500_CHECK_HIP(hipMemcpy(col, g->col.get(), g->m * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost));
+    if (deg_new) {
+      std::vector<uint32_t> d((size_t)g->n);
+      G500_CHECK_HIP(hipMemcpy(d.data(), g->deg_new.get(), g->n * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < g->n; ++i) deg_new[i] = d[(size_t)i];
+    }
+    if (new2orig)
+      G500_CHECK_HIP(hipMemcpy(new2orig, g->new2orig.get(), g->n * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost));
+    if (orig2new)
+      G500_CHECK_HIP(hipMemcpy(orig2new, g->orig2new.get(), g->n * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost));
+  });
+}
+
+// mpi/benchmark_helper.hpp:527-562 at one rank.
+int g500_find_roots(g500_graph* g, int nroots, int r1, int r2, int64_t* roots,
+                    int* found) {
+  return guard([&] {
+    need_graph(g, true);
+    if (nroots < 0 || (nroots && !roots) || !found) g500::fail(G500_ERR_ARG, "bad arguments");
+    use_device(g->device);
+    const int64_t nglobal = (int64_t)1 << g->log_n;
+    int64_t counter = 0;
+    int idx;
+    for (idx = 0; idx < nroots; ++idx) {
+      int64_t root;
+      for (;;) {
+        double d[2];
+        g500::make_random_numbers_host(2, (uint64_t)(2 + r1), (uint64_t)(3 + r2), counter, d);
+        root = (int64_t)((d[0] + d[1]) * (double)nglobal) % nglobal;
+        counter += 2;
+        if (counter > 2 * nglobal) break;
+        bool dup = false;
+        for (int i = 0; i < idx; ++i)
+          if (roots[i] == root) {
+            dup = true;
+            break;
+          }
+        if (dup) continue;
+        if (has_edge_host(g, root)) break;
+      }
+      roots[idx] = root;
+    }
+    *found = idx;
+  });
+}
+
+int g500_graph_free(g500_graph* g) {
+  return guard([&] {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->stream);
+    delete g;
+  });
+}
+
+int g500_bfs_create(g500_graph* g, g500_bfs** out) {
+  return guard([&] {
+    need_graph(g, true);
+    if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    use_device(g->device);
+    auto* b = new g500_bfs;
+    try {
+      b->g = g;
+      G500_CHECK_HIP(hipStreamCreate(&b->stream));
+      g500::bfs_init(b);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int g500_bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
+                 double beta, int64_t* at_data) {
+  return guard([&] {
+    if (!b) g500::fail(G500_ERR_ARG, "null bfs handle");
+    use_device(b->g->device);
+    g500::bfs_run(b, root, edgefactor, alpha, beta, at_data);
+  });
+}
+
+int g500_bfs_get_pred(g500_bfs* b, int64_t* pred) {
+  return guard([&] {
+    if (!b || !pred) g500::fail(G500_ERR_ARG, "null argument");
+    if (!b->have_result) g500::fail(G500_ERR_STATE, "no BFS result");
+    use_device(b->g->device);
+    G500_CHECK_HIP(hipMemcpyAsync(pred, b->pred.get(), b->g->n * sizeof(int64_t),
+                                  hipMemcpyDeviceToHost, b->stream));
+    G500_CHECK_HIP(hipStreamSynchronize(b->stream));
+  });
+}
+
+int g500_bfs_get_levels(g500_bfs* b, uint16_t* levels) {
+  return guard([&] {
+    if (!b || !levels) g500::fail(G500_ERR_ARG, "null argument");
+    if (!b->have_result) g500::fail(G500_ERR_STATE, "no BFS result");
+    use_device(b->g->device);
+    g500::bfs_levels(b, levels);
+  });
+}
+
+int g500_bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
+                      int64_t* te, int64_t* errs, int* ok) {
+  return guard([&] {
+    if (!b || !te || !errs || !ok) g500::fail(G500_ERR_ARG, "null argument");
+    use_device(b->g->device);
+    g500::bfs_validate(b, root, host_pred, te, errs, ok);
+  });
+}
+
+int g500_bfs_set_timing(g500_bfs* b, int on) {
+  return guard([&] {
+    if (!b) g500::fail(G500_ERR_ARG, "null bfs handle");
+    b->timing = on != 0;
+  });
+}
+
+int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st, g500_level_trace* levels,
+                       int max_levels) {
+  return guard([&] {
+    if (!b) g500::fail(G500_ERR_ARG, "null bfs handle");
+    if (st) *st = b->stats;
+    if (levels)
+      for (int i = 0; i < max_levels && i < (int)b->trace.size(); ++i)
+        levels[i] = b->trace[(size_t)i];
+  });
+}
+
+int g500_bfs_destroy(g500_bfs* b) {
+  return guard([&] {
+    if (!b) return;
+    (void)hipSetDevice(b->g->device);
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+  });
+}
+
+const char* g500_last_error(void) { return t_last_error.c_str(); }
+
+}  // extern "C"

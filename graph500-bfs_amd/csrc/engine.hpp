@@ -1,0 +1,44 @@
+// engine.hpp -- the BFS engine handle (one per concurrent BFS stream).
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+
+struct g500_bfs {
+  g500_graph* g = nullptr;
+  hipStream_t stream = nullptr;
+
+  int64_t words = 0;                // bitmap words covering [0, n_c)
+  g500::DevBuf<int64_t> pred;       // n, by original ID (packed entries)
+  g500::DevBuf<uint64_t> vis, vis_alt;  // visited bitmaps (new IDs)
+  g500::DevBuf<uint32_t> q, nq;     // frontier queues (new IDs)
+  g500::DevBuf<uint64_t> dsum;      // frontier degree prefix sums
+  g500::DevBuf<unsigned long long> counters;  // per-level counters
+  g500::ScanWorkspace scan_ws;
+
+  // validation scratch
+  g500::DevBuf<unsigned long long> vcount;  // [G500_VERR_NUM + 2]
+  g500::DevBuf<uint8_t> pred_valid;         // n
+
+  unsigned long long* h_counters = nullptr;  // pinned
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
+  bool timing = false;
+
+  bool have_result = false;
+  int64_t last_root = -1;
+  g500_run_stats stats{};
+  std::vector<g500_level_trace> trace;
+
+  ~g500_bfs();
+};
+
+namespace g500 {
+void bfs_init(g500_bfs* b);
+void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
+             double beta, int64_t* at_data);
+void bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
+                  int64_t* te, int64_t* errs, int* ok);
+void bfs_levels(g500_bfs* b, uint16_t* out);
+}  // namespace g500

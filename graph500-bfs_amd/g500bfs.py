@@ -1,0 +1,234 @@
+"""ctypes binding of libg500bfs_hip.so (C ABI in include/g500bfs.h).
+
+Thin plumbing for the test-suite and bench.py: every call goes straight to
+the HIP library; there is no CPU fallback.  A missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libg500bfs_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "g500bfs.h")
+
+AT_NUM = 9
+VERR_NUM = 8
+VERR_NAMES = ["range", "root", "self_parent", "depth_map", "edge_tree",
+              "edge_gap", "pred_depth", "no_parent_edge"]
+
+
+class G500Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"g500 error {code}: {msg}")
+        self.code = code
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("log_n", C.c_int), ("corebfs", C.c_int), ("n", C.c_int64),
+                ("n_c", C.c_int64), ("m", C.c_int64), ("m_sym", C.c_int64),
+                ("nedges_input", C.c_int64), ("max_used_vertex", C.c_int64),
+                ("max_degree", C.c_int64), ("core_vertices", C.c_int64),
+                ("tree_vertices", C.c_int64), ("device_bytes", C.c_int64)]
+
+
+class LevelTrace(C.Structure):
+    _fields_ = [("level", C.c_int), ("top_down", C.c_int), ("nq_size", C.c_int64),
+                ("nq_edges", C.c_int64), ("gpu_ms", C.c_float)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("nlevels", C.c_int), ("gpu_ms", C.c_double), ("td_ms", C.c_double),
+                ("bu_ms", C.c_double), ("td_edges", C.c_int64), ("root_level", C.c_int64)]
+
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: build it (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i64p = C.c_void_p, C.c_int64, C.POINTER(C.c_int64)
+    u32p = C.POINTER(C.c_uint32)
+    sig = {
+        "g500_graph_generate": [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.POINTER(vp)],
+        "g500_graph_from_edges": [C.c_int, i64p, i64p, i64, C.POINTER(vp)],
+        "g500_graph_construct": [vp, C.c_int],
+        "g500_graph_get_info": [vp, C.POINTER(GraphInfo)],
+        "g500_graph_has_edge": [vp, i64, C.POINTER(C.c_int)],
+        "g500_graph_copy_edges": [vp, i64, i64, i64p, i64p],
+        "g500_graph_copy_csr": [vp, i64p, u32p, i64p, u32p, u32p],
+        "g500_find_roots": [vp, C.c_int, C.c_int, C.c_int, i64p, C.POINTER(C.c_int)],
+        "g500_graph_free": [vp],
+        "g500_bfs_create": [vp, C.POINTER(vp)],
+        "g500_bfs_run": [vp, i64, C.c_int, C.c_double, C.c_double, i64p],
+        "g500_bfs_get_pred": [vp, i64p],
+        "g500_bfs_get_levels": [vp, C.POINTER(C.c_uint16)],
+        "g500_bfs_validate": [vp, i64, i64p, i64p, i64p, C.POINTER(C.c_int)],
+        "g500_bfs_set_timing": [vp, C.c_int],
+        "g500_bfs_get_stats": [vp, C.POINTER(RunStats), C.POINTER(LevelTrace), C.c_int],
+        "g500_bfs_destroy": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.g500_last_error.argtypes = []
+    L.g500_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise G500Error(rc, load().g500_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Graph:
+    """A device-resident Graph500 graph (edge list + CSR)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        self._info = None
+
+    @classmethod
+    def generate(cls, scale, edgefactor=16, seed1=2, seed2=3, device=0):
+        h = C.c_void_p()
+        _check(load().g500_graph_generate(device, scale, edgefactor, seed1, seed2, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, v0, v1, device=0):
+        v0 = np.ascontiguousarray(v0, np.int64)
+        v1 = np.ascontiguousarray(v1, np.int64)
+        h = C.c_void_p()
+        _check(load().g500_graph_from_edges(device, _p(v0, C.c_int64), _p(v1, C.c_int64),
+                                            len(v0), C.byref(h)))
+        return cls(h)
+
+    def construct(self, corebfs=False):
+        _check(load().g500_graph_construct(self.h, int(corebfs)))
+        self._info = None
+        return self
+
+    @property
+    def info(self):
+        if self._info is None:
+            inf = GraphInfo()
+            _check(load().g500_graph_get_info(self.h, C.byref(inf)))
+            self._info = inf
+        return self._info
+
+    def has_edge(self, v):
+        out = C.c_int()
+        _check(load().g500_graph_has_edge(self.h, v, C.byref(out)))
+        return bool(out.value)
+
+    def edges(self, start=0, count=None):
+        if count is None:
+            count = self.info.nedges_input - start
+        v0 = np.empty(count, np.int64)
+        v1 = np.empty(count, np.int64)
+        _check(load().g500_graph_copy_edges(self.h, start, count, _p(v0, C.c_int64),
+                                            _p(v1, C.c_int64)))
+        return v0, v1
+
+    def csr(self):
+        inf = self.info
+        row = np.empty(inf.n + 1, np.int64)
+        col = np.empty(max(inf.m, 1), np.uint32)
+        deg = np.empty(inf.n, np.int64)
+        n2o = np.empty(inf.n, np.uint32)
+        o2n = np.empty(inf.n, np.uint32)
+        _check(load().g500_graph_copy_csr(self.h, _p(row, C.c_int64), _p(col, C.c_uint32),
+                                          _p(deg, C.c_int64), _p(n2o, C.c_uint32),
+                                          _p(o2n, C.c_uint32)))
+        return dict(row_off=row, col=col[:inf.m], deg_new=deg, new2orig=n2o, orig2new=o2n)
+
+    def find_roots(self, nroots, r1=0, r2=0):
+        out = np.empty(max(nroots, 1), np.int64)
+        found = C.c_int()
+        _check(load().g500_find_roots(self.h, nroots, r1, r2, _p(out, C.c_int64),
+                                      C.byref(found)))
+        return out[:found.value].copy()
+
+    def free(self):
+        if self.h:
+            _check(load().g500_graph_free(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Bfs:
+    """BFS engine on a constructed graph (BfsBase's prepare/run/get_pred/end)."""
+
+    def __init__(self, graph):
+        self.graph = graph
+        h = C.c_void_p()
+        _check(load().g500_bfs_create(graph.h, C.byref(h)))
+        self.h = h
+
+    def run(self, root, edgefactor=16, alpha=10.0, beta=14.0):
+        at = np.full(AT_NUM, -1, np.int64)
+        _check(load().g500_bfs_run(self.h, int(root), edgefactor, alpha, beta,
+                                   _p(at, C.c_int64)))
+        return at
+
+    def pred(self):
+        out = np.empty(self.graph.info.n, np.int64)
+        _check(load().g500_bfs_get_pred(self.h, _p(out, C.c_int64)))
+        return out
+
+    def levels(self):
+        out = np.empty(self.graph.info.n, np.uint16)
+        _check(load().g500_bfs_get_levels(self.h, _p(out, C.c_uint16)))
+        return out
+
+    def validate(self, root, host_pred=None):
+        te = C.c_int64()
+        errs = np.zeros(VERR_NUM, np.int64)
+        ok = C.c_int()
+        hp = None
+        if host_pred is not None:
+            host_pred = np.ascontiguousarray(host_pred, np.int64)
+            hp = _p(host_pred, C.c_int64)
+        _check(load().g500_bfs_validate(self.h, int(root), hp, C.byref(te),
+                                        _p(errs, C.c_int64), C.byref(ok)))
+        return bool(ok.value), te.value, dict(zip(VERR_NAMES, errs.tolist()))
+
+    def set_timing(self, on=True):
+        _check(load().g500_bfs_set_timing(self.h, int(on)))
+
+    def stats(self, max_levels=256):
+        st = RunStats()
+        tr = (LevelTrace * max_levels)()
+        _check(load().g500_bfs_get_stats(self.h, C.byref(st), tr, max_levels))
+        levels = [dict(level=tr[i].level, top_down=tr[i].top_down, nq_size=tr[i].nq_size,
+                       nq_edges=tr[i].nq_edges, gpu_ms=tr[i].gpu_ms)
+                  for i in range(min(st.nlevels, max_levels))]
+        return st, levels
+
+    def destroy(self):
+        if self.h:
+            _check(load().g500_bfs_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

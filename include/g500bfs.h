@@ -1,0 +1,190 @@
+/*
+ * g500bfs.h -- C ABI of the MI355X direction-optimizing BFS.
+ *
+ * Drop-in boundary for the single-rank path of RIKEN-RCCS/Graph500-BFS.
+ * The reference has no runtime plugin interface: its seam is the compile-time
+ * typedef `BfsOnCPU` (mpi/bfs_cpu.hpp:13) and the driver calls the BfsBase
+ * method set (mpi/bfs.hpp:116-172).  Each entry point below names the
+ * reference interface it replaces.  The C++ class `BfsOnMI355X`
+ * (graph500-bfs_amd/host/bfs_mi355x.hpp) wraps these calls with exactly that
+ * method set.
+ *
+ * Conventions
+ *  - Every call returns G500_OK (0) or a negative G500_ERR_* code; the message
+ *    of the last failure on the calling thread is g500_last_error().
+ *  - Host buffers belong to the caller; device buffers belong to the handle.
+ *  - Vertex IDs at the boundary are the generator's ("original") IDs, as in
+ *    the reference.  A pred entry packs parent (bits 0-47, signed) and BFS
+ *    depth (bits 48-63, 0xFFFF = unreached); -1 = unreached
+ *    (mpi/validate.hpp:982-992, mpi/bfs.hpp:1457).
+ *  - One BFS at a time per g500_bfs handle (as BfsBase); handles on the same
+ *    graph may run concurrently on their own streams.
+ */
+#ifndef G500BFS_H
+#define G500BFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  G500_OK = 0,
+  G500_ERR_ARG = -1,     /* invalid argument */
+  G500_ERR_HIP = -2,     /* HIP runtime failure */
+  G500_ERR_NOMEM = -3,   /* device or host allocation failed */
+  G500_ERR_STATE = -4,   /* call out of order (e.g. run before construct) */
+  G500_ERR_NODEV = -5    /* no usable GPU */
+};
+
+/* auto_tuning_data slots, mpi/parameters.h:260-269 */
+enum {
+  G500_AT_GLOBAL_NQ_EDGES = 0,
+  G500_AT_NUM_GLOBAL_EDGES = 1,
+  G500_AT_T2B_LEVEL = 2,
+  G500_AT_GLOBAL_NQ_SIZE = 3,
+  G500_AT_NUM_GLOBAL_VERTS = 4,
+  G500_AT_B2T_LEVEL = 5,
+  G500_AT_PRE_GLOBAL_NQ_EDGES = 6,
+  G500_AT_PRE_GLOBAL_NQ_SIZE = 7,
+  G500_AT_LEVEL = 8,
+  G500_AT_NUM = 9
+};
+
+/* validation error classes, in the order mpi/validate.hpp:608-976 checks them */
+enum {
+  G500_VERR_RANGE = 0,
+  G500_VERR_ROOT = 1,
+  G500_VERR_SELF_PARENT = 2,
+  G500_VERR_DEPTH_MAP = 3,
+  G500_VERR_EDGE_TREE = 4,
+  G500_VERR_EDGE_GAP = 5,
+  G500_VERR_PRED_DEPTH = 6,
+  G500_VERR_NO_PARENT_EDGE = 7,
+  G500_VERR_NUM = 8
+};
+
+typedef struct g500_graph g500_graph;
+typedef struct g500_bfs g500_bfs;
+
+typedef struct g500_graph_info {
+  int log_n;               /* estimated SCALE, Graph2DCSR::log_orig_global_verts_ */
+  int corebfs;             /* 1 if constructed with CoreBFS pruning (-C) */
+  int64_t n;               /* 2^log_n = pred length, num_orig_local_verts_ */
+  int64_t n_c;             /* vertices with >= 1 CSR edge, num_global_verts_ */
+  int64_t m;               /* deduplicated directed CSR edges, num_global_edges_ */
+  int64_t m_sym;           /* symmetric edges before dedupe */
+  int64_t nedges_input;    /* generated / uploaded edges (validation input) */
+  int64_t max_used_vertex; /* find_max_used_vertex, benchmark_helper.hpp:564-582 */
+  int64_t max_degree;      /* largest symmetric degree */
+  int64_t core_vertices;   /* CoreBFS: vertices kept in the 2-core, else 0 */
+  int64_t tree_vertices;   /* CoreBFS: pruned vertices holding a parent */
+  int64_t device_bytes;    /* device memory held by the graph */
+} g500_graph_info;
+
+/* ---- graph (L3/L4: generation + construction) -------------------------- */
+
+/* Kronecker (R-MAT A/B/C/D = .57/.19/.19/.05) edge list generated on the GPU.
+ * Replaces generate_graph_spec2010 (mpi/benchmark_helper.hpp:234-241 ->
+ * generator/graph_generator.hpp:781-892).  seed1/seed2 = PRM::USERSEED1/2
+ * (2, 3 in the reference, mpi/parameters.h:204-205). */
+int g500_graph_generate(int device, int scale, int edgefactor, uint64_t seed1,
+                        uint64_t seed2, g500_graph** out);
+
+/* Takes an existing edge list (original IDs, < 2^32) instead of generating
+ * one; nedges may be 0.  Replaces filling EdgeListStorage
+ * (generator/graph_generator.hpp:46-485) from another source. */
+int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
+                          int64_t nedges, g500_graph** out);
+
+/* Symmetrize, relabel by degree and build the CSR on the GPU; with
+ * corebfs != 0 also prune trees (CoreBFS) first.  Replaces
+ * make_symmetry_edge_list (mpi/benchmark_helper.hpp:389-497) +
+ * BfsBase::construct (mpi/bfs.hpp:116-139). */
+int g500_graph_construct(g500_graph* g, int corebfs);
+
+int g500_graph_get_info(const g500_graph* g, g500_graph_info* info);
+
+/* BfsBase::has_edge (mpi/bfs.hpp:167-170). */
+int g500_graph_has_edge(g500_graph* g, int64_t v, int* out);
+
+/* Copies input edges [start, start+count) to host int64 arrays. */
+int g500_graph_copy_edges(const g500_graph* g, int64_t start, int64_t count,
+                          int64_t* v0, int64_t* v1);
+
+/* Copies the constructed CSR to the host (any pointer may be NULL).
+ * row_off: n+1 int64 by new ID; col: m uint32 new IDs; deg_new: n int64
+ * symmetric degree by new ID; new2orig / orig2new: n uint32. */
+int g500_graph_copy_csr(const g500_graph* g, int64_t* row_off, uint32_t* col,
+                        int64_t* deg_new, uint32_t* new2orig,
+                        uint32_t* orig2new);
+
+/* Roots drawn as find_roots does (mpi/benchmark_helper.hpp:527-562) with
+ * seeds (seed1 + r1, seed2 + r2); returns the count in *found. */
+int g500_find_roots(g500_graph* g, int nroots, int r1, int r2, int64_t* roots,
+                    int* found);
+
+int g500_graph_free(g500_graph* g);
+
+/* ---- BFS engine (L5) --------------------------------------------------- */
+
+/* BfsBase::prepare_bfs -> allocate_memory (mpi/bfs.hpp:141-151, 190-303);
+ * with CoreBFS also corebfs_index::reset_unreachable
+ * (mpi/corebfs_adaptor.hpp:228-249). */
+int g500_bfs_create(g500_graph* g, g500_bfs** out);
+
+/* BfsBase::run_bfs(root, pred, ef, alpha, beta, auto_tuning_data)
+ * (mpi/bfs.hpp:3214-3232).  pred stays on the device until
+ * g500_bfs_get_pred; at_data may be NULL.  Blocks until the BFS finished. */
+int g500_bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
+                 double beta, int64_t* at_data);
+
+/* BfsBase::get_pred (mpi/bfs.hpp:161-163): device pred -> host, n entries. */
+int g500_bfs_get_pred(g500_bfs* b, int64_t* pred);
+
+/* Depth of every vertex by original ID (pred >> 48), n entries. */
+int g500_bfs_get_levels(g500_bfs* b, uint16_t* levels);
+
+/* validate_bfs_result (mpi/validate.hpp:1269-1277) on the device against
+ * the resident input edge list, using the last run's pred (or, when
+ * host_pred != NULL, that array uploaded first).  nglobalverts is
+ * max_used_vertex+1 as in mpi/main.cc:646.  Fills te (traversed edges),
+ * errs[G500_VERR_NUM] and *ok.  With CoreBFS the rebuilt depths are written
+ * back into the device pred, as the reference does in place. */
+int g500_bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
+                      int64_t* te, int64_t* errs, int* ok);
+
+/* Per-level trace of the last run. */
+typedef struct g500_level_trace {
+  int level;        /* BFS level number (root = level 0 of the core BFS) */
+  int top_down;     /* 1 top-down, 0 bottom-up */
+  int64_t nq_size;  /* vertices discovered at this level */
+  int64_t nq_edges; /* sum of their degrees (top-down levels only) */
+  float gpu_ms;     /* device time of the level, 0 unless timing is on */
+} g500_level_trace;
+
+typedef struct g500_run_stats {
+  int nlevels;          /* executed levels (incl. the final empty one) */
+  double gpu_ms;        /* device time of the whole run (events on stream) */
+  double td_ms, bu_ms;  /* device time in top-down / bottom-up levels */
+  int64_t td_edges;     /* edges scanned by top-down levels */
+  int64_t root_level;   /* level the core BFS started at (CoreBFS) */
+} g500_run_stats;
+
+/* Enables per-level HIP-event timing (off by default; costs one event pair
+ * per level). */
+int g500_bfs_set_timing(g500_bfs* b, int on);
+int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st,
+                       g500_level_trace* levels, int max_levels);
+
+/* BfsBase::end_bfs (mpi/bfs.hpp:165): frees the engine. */
+int g500_bfs_destroy(g500_bfs* b);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* g500_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* G500BFS_H */

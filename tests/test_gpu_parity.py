@@ -1,0 +1,193 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Integer work, so every comparison is bit-exact: generated edges, the CSR,
+per-root BFS levels, roots, the direction schedule and the validator's
+verdict.  Parents may legally differ from the oracle's, so parent trees are
+checked with both validators instead.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def G():
+    import g500bfs
+    g500bfs.load()
+    return g500bfs
+
+
+def _oracle_levels(og, root):
+    lev = og.bfs(int(root))
+    return np.where(lev < 0, 0xFFFF, lev).astype(np.uint16)
+
+
+@pytest.mark.parametrize("scale", [1, 6, 12, 16])
+def test_generator_bit_exact(G, scale):
+    print(f"\n[gen] scale={scale}", flush=True)
+    g = G.Graph.generate(scale)
+    d0, d1 = g.edges()
+    o0, o1 = O.generate_edges(scale)
+    assert np.array_equal(d0, o0) and np.array_equal(d1, o1)
+
+
+@pytest.mark.parametrize("scale", [6, 12, 16])
+def test_csr_bit_exact(G, scale):
+    print(f"\n[csr] scale={scale}", flush=True)
+    g = G.Graph.generate(scale).construct()
+    o0, o1 = O.generate_edges(scale)
+    og = O.Graph(o0, o1)
+    inf = g.info
+    assert (inf.log_n, inf.n, inf.n_c, inf.m, inf.m_sym, inf.max_used_vertex) == (
+        og.info.log_n, og.info.n, og.info.n_c, og.info.m, og.info.m_sym,
+        og.info.max_used_vertex)
+    a, b = g.csr(), og.arrays()
+    for k in ("row_off", "col", "deg_new", "new2orig", "orig2new"):
+        assert np.array_equal(a[k].astype(np.int64), b[k].astype(np.int64)), k
+
+
+def _check_roots(G, g, og, roots, golden_te=None, alpha=10.0, beta=14.0):
+    bfs = G.Bfs(g)
+    for r in roots:
+        r = int(r)
+        at = bfs.run(r, alpha=alpha, beta=beta)
+        lev = bfs.levels()
+        assert np.array_equal(lev, _oracle_levels(og, r)), f"levels differ for root {r}"
+        ok, te, errs = bfs.validate(r)
+        assert ok, (r, errs)
+        pred = bfs.pred()
+        ok2, te2, errs2 = og.validate(r, pred)
+        assert ok2, (r, errs2)
+        assert te == te2
+        if golden_te is not None:
+            assert te == golden_te
+        # same direction schedule and auto-tuning slots as the reference loop
+        _, at_o, dirs_o = og.do_bfs(r, alpha=alpha, beta=beta)
+        st, trace = bfs.stats()
+        assert [t["top_down"] for t in trace] == dirs_o.tolist()
+        assert at.tolist() == at_o.tolist()
+    return bfs
+
+
+def test_bfs_scale16_reference_roots(G, golden):
+    print("\n[bfs] scale=16 roots=16", flush=True)
+    g = G.Graph.generate(16).construct()
+    o0, o1 = O.generate_edges(16)
+    og = O.Graph(o0, o1)
+    roots = g.find_roots(16)
+    assert roots.tolist() == og.find_roots(16).tolist()
+    _check_roots(G, g, og, roots, golden_te=golden["pf_nedge"][16])
+
+
+def test_bfs_scale20_te(G, golden):
+    print("\n[bfs] scale=20 roots=4", flush=True)
+    g = G.Graph.generate(20).construct()
+    inf = g.info
+    ref = golden["edge_counts"]["20"]
+    assert (inf.m_sym, inf.m, inf.n_c) == (ref["sym"], ref["dedup"], ref["n_c"])
+    o0, o1 = O.generate_edges(20)
+    og = O.Graph(o0, o1)
+    roots = g.find_roots(4)
+    _check_roots(G, g, og, roots, golden_te=golden["pf_nedge"][20])
+
+
+@pytest.mark.parametrize("alpha,beta", [(1e9, 14.0), (1e-3, 14.0), (10.0, 1e-6), (0.5, 1e6)])
+def test_direction_extremes(G, alpha, beta):
+    """Pure top-down, immediate bottom-up, never-return and eager-return."""
+    print(f"\n[dir] alpha={alpha} beta={beta}", flush=True)
+    g = G.Graph.generate(14).construct()
+    o0, o1 = O.generate_edges(14)
+    og = O.Graph(o0, o1)
+    _check_roots(G, g, og, g.find_roots(3), alpha=alpha, beta=beta)
+
+
+def _small_cases():
+    rng = np.random.default_rng(5)
+    path = np.arange(9)
+    star_c = np.zeros(30, np.int64)
+    two = (np.array([0, 1, 2, 10, 11]), np.array([1, 2, 0, 11, 12]))
+    dup = (np.array([3, 3, 3, 4, 4, 5]), np.array([4, 4, 4, 3, 5, 5]))
+    rnd = rng.integers(0, 300, size=(2, 900))
+    return {
+        "path": (path[:-1], path[1:]),
+        "star": (star_c, np.arange(1, 31)),
+        "two_components": two,
+        "dups_and_loops": dup,
+        "random_sparse": (rnd[0], rnd[1]),
+        "single_edge": (np.array([7]), np.array([2])),
+    }
+
+
+@pytest.mark.parametrize("name", list(_small_cases()))
+def test_small_graphs(G, name):
+    print(f"\n[small] {name}", flush=True)
+    v0, v1 = _small_cases()[name]
+    v0 = np.asarray(v0, np.int64)
+    v1 = np.asarray(v1, np.int64)
+    g = G.Graph.from_edges(v0, v1).construct()
+    og = O.Graph(v0, v1)
+    assert (g.info.n, g.info.n_c, g.info.m) == (og.info.n, og.info.n_c, og.info.m)
+    a, b = g.csr(), og.arrays()
+    for k in ("row_off", "col", "new2orig"):
+        assert np.array_equal(a[k].astype(np.int64), b[k].astype(np.int64)), k
+    roots = [v for v in range(og.info.n) if og.has_edge(v)][:5]
+    _check_roots(G, g, og, roots)
+
+
+def test_degenerate_inputs(G):
+    print("\n[degenerate]", flush=True)
+    # empty edge list and an all-self-loop list build an edgeless graph
+    for v0, v1 in ((np.zeros(0, np.int64), np.zeros(0, np.int64)),
+                   (np.array([4, 4, 9]), np.array([4, 4, 9]))):
+        g = G.Graph.from_edges(v0, v1).construct()
+        assert g.info.n_c == 0 and g.info.m == 0
+        assert g.find_roots(4).tolist() == O.Graph(v0, v1).find_roots(4).tolist()
+        bfs = G.Bfs(g)
+        bfs.run(0)
+        assert bfs.pred()[0] == 0  # root is its own parent, nothing else
+    with pytest.raises(G.G500Error):
+        G.Graph.from_edges(np.array([-1]), np.array([2]))
+    g = G.Graph.generate(8).construct()
+    with pytest.raises(G.G500Error):
+        G.Bfs(g).run(1 << 20)
+
+
+def test_validator_agrees_on_planted_mistakes(G):
+    print("\n[validator] planted", flush=True)
+    g = G.Graph.generate(12).construct()
+    o0, o1 = O.generate_edges(12)
+    og = O.Graph(o0, o1)
+    r = int(g.find_roots(1)[0])
+    bfs = G.Bfs(g)
+    bfs.run(r)
+    pred = bfs.pred()
+    lev = og.bfs(r)
+    reached = np.nonzero(lev > 0)[0]
+    parents = {int((p << 16) >> 16) for p in pred[lev > 0]}
+    v = int([x for x in reached if int(x) not in parents][0])
+
+    def ent(parent, depth):
+        u = (int(depth) << 48) | (int(parent) & 0xFFFFFFFFFFFF)
+        return np.int64(u - (1 << 64) if u >= 1 << 63 else u)
+
+    plants = []
+    p = pred.copy(); p[v] = ent(og.info.max_used_vertex + 3, lev[v]); plants.append(p)
+    p = pred.copy(); p[r] = ent(v, 0); plants.append(p)
+    p = pred.copy(); p[v] = ent(v, lev[v]); plants.append(p)
+    p = pred.copy(); p[v] = ent((int(pred[v]) << 16) >> 16, lev[v] + 1); plants.append(p)
+    p = pred.copy(); p[v] = -1; plants.append(p)
+    far = [int(w) for w in np.nonzero(lev == lev[v] - 1)[0]]
+    nbrs = set(o1[o0 == v].tolist()) | set(o0[o1 == v].tolist())
+    far = [w for w in far if w not in nbrs]
+    p = pred.copy(); p[v] = ent(far[0], lev[v]); plants.append(p)
+    for p in plants:
+        ok_d, te_d, e_d = bfs.validate(r, host_pred=p)
+        ok_o, te_o, e_o = og.validate(r, p)
+        assert not ok_d and not ok_o
+        first = lambda e: next(k for k in G.VERR_NAMES if e[k])  # noqa: E731
+        assert first(e_d) == first(e_o), (e_d, e_o)
+    ok, te, _ = bfs.validate(r, host_pred=pred)
+    assert ok and te == 65535
