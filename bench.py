@@ -1,0 +1,240 @@
+"""Graph500 BFS benchmark on one MI355X: harmonic-mean GTEPS at SCALE 26.
+
+A step is one direction-optimizing BFS from one root of the Kronecker
+SCALE-26 graph (edgefactor 16, seed (2,3)), as the reference times
+`run_bfs` (mpi/main.cc:618-626).  Roots come from the reference's root rule
+(find_roots + redraw when a root reaches <= 3 levels, mpi/main.cc:334-359).
+The graph is generated and built on the GPU before timing; pred stays on the
+device during the timed BFS (get_pred is outside the timer, as in the
+reference).  Every timed root is validated afterwards on the device and its
+traversed-edge count (TE) is the TEPS numerator.
+
+Prints one JSON line (see DESIGN.md for the roofline yardstick).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "graph500-bfs_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64, help="timed BFS runs (one root each)")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--edgefactor", type=int, default=16)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--alpha", type=float, default=10.0)
+    ap.add_argument("--beta", type=float, default=14.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="budget for the CPU baseline sample (0 disables)")
+    ap.add_argument("--no-validate", action="store_true",
+                    help="skip per-root validation (TE then comes from pf_nedge)")
+    ap.add_argument("--levels", action="store_true", help="print per-level trace of one root")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+PF_NEDGE = None
+
+
+def pf_nedge(scale):
+    global PF_NEDGE
+    if PF_NEDGE is None:
+        with open(os.path.join(ROOT, "tests", "golden", "reference_counts.json")) as f:
+            PF_NEDGE = json.load(f)["pf_nedge"]
+    return PF_NEDGE[scale] if scale < len(PF_NEDGE) else None
+
+
+def cpu_baseline(graph, roots, te_by_root, budget_s, args):
+    """Oracle direction-optimizing BFS (serial C port) on the same graph."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+    import oracle_lib as O
+    inf = graph.info
+    csr = graph.csr()
+    og = O.lib().orc_graph_wrap(
+        inf.log_n, inf.n_c, inf.m,
+        csr["row_off"].ctypes.data_as(C.POINTER(C.c_int64)),
+        csr["col"].ctypes.data_as(C.POINTER(C.c_uint32)),
+        csr["deg_new"].ctypes.data_as(C.POINTER(C.c_int64)),
+        csr["new2orig"].ctypes.data_as(C.POINTER(C.c_uint32)),
+        csr["orig2new"].ctypes.data_as(C.POINTER(C.c_uint32)))
+    pred = np.empty(inf.n, np.int64)
+    at = np.empty(9, np.int64)
+    dirs = np.zeros(256, np.int8)
+    times, tes = [], []
+    t_start = time.time()
+    for r in roots:
+        t0 = time.perf_counter()
+        O.lib().orc_do_bfs(og, int(r), args.edgefactor, args.alpha, args.beta,
+                           pred.ctypes.data_as(C.POINTER(C.c_int64)),
+                           at.ctypes.data_as(C.POINTER(C.c_int64)),
+                           dirs.ctypes.data_as(C.POINTER(C.c_int8)), 256)
+        times.append(time.perf_counter() - t0)
+        tes.append(te_by_root[int(r)])
+        if time.time() - t_start > budget_s:
+            break
+    O.lib().orc_graph_free(og)
+    hm = len(times) / sum(t / e for t, e in zip(times, tes))
+    return {"value": hm / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} root(s) of the same SCALE {args.scale} graph, serial C "
+                      f"direction-optimizing BFS (oracle/g500_oracle.c orc_do_bfs), "
+                      f"{sum(times):.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not built.", flush=True)
+        sys.exit(2)
+    import g500bfs
+    g500bfs.load()
+
+    t0 = time.perf_counter()
+    graph = g500bfs.Graph.generate(args.scale, args.edgefactor, 2, 3)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    graph.construct()
+    t_con = time.perf_counter() - t0
+    inf = graph.info
+    log(f"graph: scale {args.scale} n={inf.n} n_c={inf.n_c} m={inf.m} "
+        f"gen {t_gen:.2f}s construct {t_con:.2f}s dev {inf.device_bytes / 2**30:.1f} GiB")
+
+    bfs = g500bfs.Bfs(graph)
+    # root selection with the reference's redraw rule (mpi/main.cc:334-359)
+    r1 = 0
+    while True:
+        roots = graph.find_roots(args.roots, r1=r1)
+        levels_ok = True
+        for r in roots:
+            at = bfs.run(int(r), args.edgefactor, args.alpha, args.beta)
+            if args.scale > 6 and at[8] <= 3:
+                levels_ok = False
+        if levels_ok:
+            break
+        r1 += 1
+        log("picking roots again")
+    for i in range(args.warmup):
+        bfs.run(int(roots[i % len(roots)]), args.edgefactor, args.alpha, args.beta)
+
+    # timed region: exactly `steps` BFS runs
+    nsteps = args.steps
+    times = np.zeros(nsteps)
+    gpu_ms = np.zeros(nsteps)
+    t_all = time.perf_counter()
+    for i in range(nsteps):
+        r = int(roots[i % len(roots)])
+        t0 = time.perf_counter()
+        bfs.run(r, args.edgefactor, args.alpha, args.beta)
+        times[i] = time.perf_counter() - t0
+        gpu_ms[i] = bfs.stats(1)[0].gpu_ms
+    t_all = time.perf_counter() - t_all
+
+    # untimed: validate each distinct timed root (reference -v 2 semantics)
+    te_by_root = {}
+    golden = pf_nedge(args.scale) if args.edgefactor == 16 else None
+    all_valid = True
+    for i in range(min(nsteps, len(roots))):
+        r = int(roots[i])
+        if args.no_validate:
+            te_by_root[r] = golden
+            continue
+        bfs.run(r, args.edgefactor, args.alpha, args.beta)
+        ok, te, errs = bfs.validate(r)
+        all_valid &= ok
+        te_by_root[r] = te
+        if golden is not None and te != golden:
+            log(f"root {r}: TE {te} != pf_nedge[{args.scale}] {golden}")
+    tes = np.array([te_by_root[int(roots[i % len(roots)])] for i in range(nsteps)], float)
+    teps_hm = nsteps / np.sum(times / tes)
+    gpu_teps_hm = nsteps / np.sum(gpu_ms * 1e-3 / tes)
+
+    # roofline yardstick for one root's BFS (same component for every root)
+    r0 = int(roots[0])
+    bfs.run(r0, args.edgefactor, args.alpha, args.beta)
+    if args.levels:
+        bfs.set_timing(True)
+        bfs.run(r0, args.edgefactor, args.alpha, args.beta)
+        st, trace = bfs.stats()
+        for t in trace:
+            log(f"  level {t['level']:2d} {'TD' if t['top_down'] else 'BU'} "
+                f"nq={t['nq_size']:>10d} {t['gpu_ms']:.3f} ms")
+        bfs.set_timing(False)
+        bfs.run(r0, args.edgefactor, args.alpha, args.beta)
+    csr = graph.csr()
+    lev = bfs.levels()
+    reached = lev != 0xFFFF
+    nn = csr["orig2new"][reached].astype(np.int64)
+    m_c = int((csr["row_off"][nn + 1] - csr["row_off"][nn]).sum())
+    n_comp = int(reached.sum())
+    b_alg = 4 * m_c + 8 * (n_comp + 1) + 12 * inf.n
+    te0 = te_by_root[r0]
+    b_per_te = b_alg / te0
+    mean_gpu_s = float(np.mean(gpu_ms)) * 1e-3
+    achieved_gbs = b_alg / mean_gpu_s / 1e9
+    del csr
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        try:
+            cpu = cpu_baseline(graph, roots[:8], te_by_root, args.cpu_seconds, args)
+        except Exception as e:  # the baseline is a report, never the product
+            log(f"cpu baseline failed: {e}")
+
+    out = {
+        "metric": "harmonic-mean GTEPS @SCALE 26, 1 GPU",
+        "value": round(teps_hm / 1e9, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_all / nsteps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (Graph500 Kronecker generator, seed (2,3), generated on device)",
+        "config": {
+            "workload": f"plain single-root direction-optimizing BFS, Kronecker SCALE {args.scale}, "
+                        f"edgefactor {args.edgefactor}, {len(roots)} roots",
+            "scale": args.scale, "edgefactor": args.edgefactor, "roots": len(roots),
+            "alpha": args.alpha, "beta": args.beta,
+            "n": inf.n, "n_c": inf.n_c, "m": inf.m,
+            "component_vertices": n_comp, "component_edges": m_c,
+            "te_per_root": te0, "all_roots_valid": bool(all_valid),
+            "gpu_event_hm_gteps": round(gpu_teps_hm / 1e9, 3),
+            "median_bfs_ms": round(float(np.median(times)) * 1e3, 4),
+            "generation_s": round(t_gen, 3), "construction_s": round(t_con, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "one root's BFS (all level kernels on the engine stream, HIP events)",
+            "bytes_per_te": round(b_per_te, 3),
+            "mean_bfs_gpu_ms": round(mean_gpu_s * 1e3, 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
