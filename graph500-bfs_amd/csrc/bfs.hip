@@ -20,7 +20,19 @@
 //                 until the first visited neighbour (early exit).  New bits
 //                 gather in LDS and are written once per word.
 //   k_bitmap_to_queue   bitmap -> queue at the bottom-up -> top-down switch.
+//   k_pred_gather after the last level: pred[o] for every original ID o,
+//                 gathered from the entries the levels wrote by new ID
+//                 (levels write densely by new ID; one coalesced pass then
+//                 produces pred in original-ID order).
+//
+// Level hand-off: the kernel that closes a level accumulates its counters in
+// device memory; its last workgroup (atomic ticket) posts the totals and a
+// sequence number to pinned host memory and clears the counters.  The host
+// spins on the sequence number, decides the next direction exactly as the
+// reference does, and launches the next level -- no copies, memsets or
+// stream synchronisations per level.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "engine.hpp"
@@ -34,6 +46,7 @@ constexpr int kWavesPerBlock = kT / kWave;
 constexpr int kTdItems = 8;
 constexpr int kTdTile = kT * kTdItems;
 constexpr int kMaxGrid = 2048;
+constexpr int kBuIlp = 4;  // vertices in flight per lane in k_bu_step
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -80,6 +93,24 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t* wsum,
   return x - v + off;
 }
 
+// Called by thread 0 of every block once its totals are in ls->cnt.
+__device__ inline void finish_level(LevelSync* ls, HostReport* hr,
+                                    unsigned long long seq) {
+  __threadfence();
+  const unsigned t = atomicAdd(&ls->done, 1u);
+  if (t != gridDim.x - 1) return;
+  __threadfence();
+  unsigned long long v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = atomicExch(&ls->cnt[i], 0ull);
+  atomicExch(&ls->done, 0u);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    __hip_atomic_store(&hr->v[i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __threadfence_system();
+  __hip_atomic_store(&hr->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Bits of vertices >= n_c are pre-set so they are never searched
 // (the reference pre-visits vertices without edges, mpi/bfs.hpp:330-334).
 __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
@@ -96,10 +127,12 @@ __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
   }
 }
 
-__global__ void k_bfs_root(int64_t* pred, int64_t root, int64_t entry,
-                           uint32_t* q, int64_t root_new, int64_t n_c) {
-  pred[root] = entry;
-  if (root_new >= 0 && root_new < n_c) q[0] = (uint32_t)root_new;
+__global__ void k_bfs_root(int64_t* pred_new, int64_t entry, uint32_t* q,
+                           int64_t root_new, int64_t n_c) {
+  if (root_new >= 0 && root_new < n_c) {
+    pred_new[root_new] = entry;
+    q[0] = (uint32_t)root_new;
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_td_degrees(const uint32_t* __restrict__ q,
@@ -129,8 +162,8 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     const uint64_t* __restrict__ dsum, const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint32_t* __restrict__ n2o,
     const uint32_t* __restrict__ deg_new, unsigned long long* vis,
-    int64_t* __restrict__ pred, uint32_t* __restrict__ nq,
-    unsigned long long* counters, uint64_t level_bits) {
+    int64_t* __restrict__ pred_new, uint32_t* __restrict__ nq, LevelSync* ls,
+    HostReport* hr, unsigned long long seq, uint64_t level_bits) {
   __shared__ uint64_t win[kTdTile];
   __shared__ uint32_t lq[kTdTile];
   __shared__ uint64_t red[kWavesPerBlock];
@@ -181,7 +214,7 @@ __global__ __launch_bounds__(kT) void k_td_expand(
           won = !(old & bit);
         }
         if (won) {
-          pred[n2o[v]] = (int64_t)((uint64_t)n2o[u] | level_bits);
+          pred_new[v] = (int64_t)((uint64_t)n2o[u] | level_bits);
           my_edges += deg_new[v];
         }
       }
@@ -197,20 +230,24 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     }
     __syncthreads();
     const uint32_t cnt = lq_n;
-    if (t == 0 && cnt) lq_base = atomicAdd(&counters[0], (unsigned long long)cnt);
+    if (t == 0 && cnt) lq_base = atomicAdd(&ls->cnt[0], (unsigned long long)cnt);
     __syncthreads();
     for (uint32_t j = t; j < cnt; j += kT) nq[lq_base + j] = lq[j];
     __syncthreads();
   }
   const uint64_t s = block_sum(my_edges, red);
-  if (t == 0 && s) atomicAdd(&counters[1], (unsigned long long)s);
+  if (t == 0) {
+    if (s) atomicAdd(&ls->cnt[1], (unsigned long long)s);
+    finish_level(ls, hr, seq);
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_bu_step(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
-    const uint32_t* __restrict__ col, const uint32_t* __restrict__ n2o,
-    int64_t* __restrict__ pred, unsigned long long* counters,
+    const uint32_t* __restrict__ col, const uint32_t* __restrict__ first,
+    const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
+    LevelSync* ls, HostReport* hr, unsigned long long seq,
     uint64_t level_bits) {
   __shared__ unsigned long long fw[kWavesPerBlock][kWave];
   __shared__ uint64_t red[kWavesPerBlock];
@@ -233,19 +270,25 @@ __global__ __launch_bounds__(kT) void k_bu_step(
     const uint32_t total = __shfl(incl, kWave - 1, kWave);
     fw[wave][lane] = 0;
     wave_sync();
-    for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
-      const uint32_t gidx = b0 + lane;
-      // owning word: largest l with excl_l <= gidx
-      int lo = 0;
+    // kBuIlp unvisited vertices per lane per round, loads issued together
+    for (uint32_t b0 = 0; b0 < total; b0 += kWave * kBuIlp) {
+      uint32_t v[kBuIlp], par[kBuIlp];
+      int wl[kBuIlp], bp[kBuIlp];
+      bool ok[kBuIlp], hit[kBuIlp];
 #pragma unroll
-      for (int step = 32; step > 0; step >>= 1) {
-        const int cand = lo + step;
-        const uint32_t e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
-        if (cand < kWave && e <= gidx) lo = cand;
-      }
-      const uint64_t wum = __shfl(um, lo, kWave);
-      const uint32_t elo = __shfl(excl, lo, kWave);
-      if (gidx < total) {
+      for (int j = 0; j < kBuIlp; ++j) {
+        const uint32_t gidx = b0 + j * kWave + lane;
+        // owning word: largest l with excl_l <= gidx
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+          const int cand = lo + step;
+          const uint32_t e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
+          if (cand < kWave && e <= gidx) lo = cand;
+        }
+        const uint64_t wum = __shfl(um, lo, kWave);
+        const uint32_t elo = __shfl(excl, lo, kWave);
+        ok[j] = gidx < total;
         uint32_t r = gidx - elo;
         uint64_t x = wum;
         int pos = 0;
@@ -258,16 +301,44 @@ __global__ __launch_bounds__(kT) void k_bu_step(
             pos += half;
           }
         }
-        const uint32_t v = (uint32_t)((c * kWave + lo) * 64 + pos);
-        const int64_t k0 = row_off[v], k1 = row_off[v + 1];
-        for (int64_t k = k0; k < k1; ++k) {
-          const uint32_t u = col[k];
-          if ((vis_old[u >> 6] >> (u & 63)) & 1ull) {
-            atomicOr(&fw[wave][lo], 1ull << pos);
-            pred[n2o[v]] = (int64_t)((uint64_t)n2o[u] | level_bits);
-            break;
-          }
+        wl[j] = lo;
+        bp[j] = pos;
+        v[j] = (uint32_t)((c * kWave + lo) * 64 + pos);
+      }
+      // first neighbour (highest degree) from the dense copy
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) par[j] = ok[j] ? first[v[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j)
+        hit[j] = ok[j] && ((vis_old[par[j] >> 6] >> (par[j] & 63)) & 1ull);
+      // rest of the row, 4 independent loads at a time
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) {
+        if (!ok[j] || hit[j]) continue;
+        int64_t k = row_off[v[j]] + 1;
+        const int64_t k1 = row_off[v[j] + 1];
+        while (!hit[j] && k < k1) {
+          uint32_t u[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) u[t] = (k + t < k1) ? col[k + t] : par[j];
+          bool h[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            h[t] = (k + t < k1) && ((vis_old[u[t] >> 6] >> (u[t] & 63)) & 1ull);
+#pragma unroll
+          for (int t = 3; t >= 0; --t)
+            if (h[t]) {
+              hit[j] = true;
+              par[j] = u[t];
+            }
+          k += 4;
         }
+      }
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) {
+        if (!hit[j]) continue;
+        atomicOr(&fw[wave][wl[j]], 1ull << bp[j]);
+        pred_new[v[j]] = (int64_t)((uint64_t)n2o[par[j]] | level_bits);
       }
     }
     wave_sync();
@@ -277,22 +348,25 @@ __global__ __launch_bounds__(kT) void k_bu_step(
     wave_sync();
   }
   const uint64_t s = block_sum(nfound, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&counters[0], (unsigned long long)s);
+  if (threadIdx.x == 0) {
+    if (s) atomicAdd(&ls->cnt[0], (unsigned long long)s);
+    finish_level(ls, hr, seq);
+  }
 }
 
-// queue <- bits set in cur but not in prev; counters[2] += count,
-// counters[3] += degree sum.
+// queue <- bits set in cur but not in prev; reports (count, degree sum).
 __global__ __launch_bounds__(kT) void k_bitmap_to_queue(
     const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
     int64_t words, const uint32_t* __restrict__ deg_new,
-    uint32_t* __restrict__ out, unsigned long long* counters) {
+    uint32_t* __restrict__ out, LevelSync* ls, HostReport* hr,
+    unsigned long long seq) {
   __shared__ uint64_t wsum[kWavesPerBlock];
   __shared__ unsigned long long base;
   const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
   uint64_t bits = w < words ? (cur[w] & ~prev[w]) : 0;
   uint64_t tot;
   const uint64_t excl = block_excl_scan((uint64_t)__popcll(bits), wsum, &tot);
-  if (threadIdx.x == 0) base = tot ? atomicAdd(&counters[2], (unsigned long long)tot) : 0;
+  if (threadIdx.x == 0) base = tot ? atomicAdd(&ls->cnt[0], (unsigned long long)tot) : 0;
   __syncthreads();
   uint64_t pos = base + excl, edges = 0;
   while (bits) {
@@ -303,7 +377,27 @@ __global__ __launch_bounds__(kT) void k_bitmap_to_queue(
     edges += deg_new[v];
   }
   const uint64_t s = block_sum(edges, wsum);
-  if (threadIdx.x == 0 && s) atomicAdd(&counters[3], (unsigned long long)s);
+  if (threadIdx.x == 0) {
+    if (s) atomicAdd(&ls->cnt[1], (unsigned long long)s);
+    finish_level(ls, hr, seq);
+  }
+}
+
+// pred[o] for every original ID: the level-written entry of its new ID if
+// that vertex was reached, the root's own entry, else -1.
+__global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__ o2n,
+                                                   const uint64_t* __restrict__ vis,
+                                                   const int64_t* __restrict__ pred_new,
+                                                   int64_t n, int64_t n_c, int64_t root,
+                                                   int64_t* __restrict__ pred) {
+  for (int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x; o < n;
+       o += (int64_t)gridDim.x * kT) {
+    const uint32_t v = o2n[o];
+    int64_t e = -1;
+    if ((int64_t)v < n_c && ((vis[v >> 6] >> (v & 63)) & 1ull)) e = pred_new[v];
+    if (o == root) e = root;
+    pred[o] = e;
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_levels(const int64_t* __restrict__ pred,
@@ -323,19 +417,49 @@ void bfs_init(g500_bfs* b) {
   g500_graph* g = b->g;
   b->words = ceil_div(std::max<int64_t>(g->n_c, 1), 64);
   b->pred.alloc((size_t)g->n);
+  b->pred_new.alloc((size_t)std::max<int64_t>(g->n_c, 1));
   b->vis.alloc((size_t)b->words);
   b->vis_alt.alloc((size_t)b->words);
   b->q.alloc((size_t)std::max<int64_t>(g->n_c, 1));
   b->nq.alloc((size_t)std::max<int64_t>(g->n_c, 1));
   b->dsum.alloc((size_t)std::max<int64_t>(g->n_c, 1) + 1);
-  b->counters.alloc(8);
+  b->lsync.alloc(1);
+  G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
   b->scan_ws.reserve(std::max<int64_t>(g->n_c, 1));
-  G500_CHECK_HIP(hipHostMalloc((void**)&b->h_counters, 8 * sizeof(unsigned long long),
-                               hipHostMallocDefault));
+  G500_CHECK_HIP(hipHostMalloc((void**)&b->h_report, sizeof(HostReport),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset((void*)b->h_report, 0, sizeof(HostReport));
+  G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
   ensure_host_orig2new(g);
 }
+
+namespace {
+
+// Spins until the GPU posted report `seq`; returns its four totals.
+void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4]) {
+  HostReport* hr = b->h_report;
+  uint64_t spins = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(&hr->seq, __ATOMIC_ACQUIRE) != seq) {
+    if ((++spins & 1023) == 0) {
+      const hipError_t q = hipStreamQuery(b->stream);
+      if (q == hipSuccess) {
+        if (__atomic_load_n(&hr->seq, __ATOMIC_ACQUIRE) == seq) break;
+        fail(G500_ERR_STATE, "level report %llu missing after the stream drained", seq);
+      }
+      if (q != hipErrorNotReady)
+        fail(G500_ERR_HIP, "stream error while waiting for a level: %s",
+             hipGetErrorString(q));
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300))
+        fail(G500_ERR_STATE, "timed out waiting for level report %llu", seq);
+    }
+  }
+  for (int i = 0; i < 4; ++i) out[i] = __atomic_load_n(&hr->v[i], __ATOMIC_ACQUIRE);
+}
+
+}  // namespace
 
 void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
              double beta, int64_t* at) {
@@ -358,24 +482,16 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->stats = g500_run_stats{};
   b->have_result = false;
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
-  G500_CHECK_HIP(hipMemsetAsync(b->pred.get(), 0xFF, g->n * sizeof(int64_t), s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1);
   G500_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, b->pred.get(), root,
-                     root /* depth 0 */, b->q.get(), root_in ? root_new : (int64_t)-1,
-                     n_c);
+  hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, b->pred_new.get(),
+                     root /* parent = root, depth 0 */, b->q.get(),
+                     root_in ? root_new : (int64_t)-1, n_c);
   G500_CHECK_LAUNCH();
 
   int64_t qlen = root_in ? 1 : 0;
-  int64_t est_edges = 0;
-  if (root_in) {
-    int64_t ro[2];
-    G500_CHECK_HIP(hipMemcpyAsync(ro, g->row_off.get() + root_new, 2 * sizeof(int64_t),
-                                  hipMemcpyDeviceToHost, s));
-    G500_CHECK_HIP(hipStreamSynchronize(s));
-    est_edges = ro[1] - ro[0];
-  }
+  int64_t est_edges = -1;  // unknown for the root: use a mid-size grid
 
   // level-loop state (mpi/bfs.hpp:3264-3279)
   bool next_td = true, growing = true;
@@ -383,7 +499,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   int level = 0;
   const int64_t alpha_thr = (int64_t)((double)g->m / alpha);
   const int64_t beta_thr = (int64_t)((double)n_c / (beta * edgefactor * 2.0));
-  unsigned long long* cnt = b->counters.get();
+  LevelSync* ls = b->lsync.get();
+  unsigned long long rep[4];
 
   for (;;) {
     at[G500_AT_LEVEL] = ++level;
@@ -400,8 +517,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       lev1 = b->level_events[k + 1];
       G500_CHECK_HIP(hipEventRecord(lev0, s));
     }
-    G500_CHECK_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), s));
     const uint64_t level_bits = (uint64_t)level << 48;
+    const unsigned long long seq = ++b->seq;
     if (td) {
       if (qlen > 0) {
         hipLaunchKernelGGL(k_td_degrees, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT),
@@ -409,37 +526,31 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
         G500_CHECK_LAUNCH();
       }
       exclusive_scan<uint64_t>(b->dsum.get(), b->dsum.get(), qlen, b->scan_ws, s);
-      hipLaunchKernelGGL(k_td_expand, dim3(grid_for(est_edges, kTdTile)), dim3(kT), 0, s,
+      const unsigned grid = est_edges < 0 ? 256u : grid_for(est_edges, kTdTile);
+      hipLaunchKernelGGL(k_td_expand, dim3(grid), dim3(kT), 0, s,
                          b->q.get(), qlen, b->dsum.get(), g->row_off.get(),
                          g->col.get(), g->new2orig.get(), g->deg_new.get(),
-                         (unsigned long long*)b->vis.get(), b->pred.get(), b->nq.get(),
-                         cnt, level_bits);
+                         (unsigned long long*)b->vis.get(), b->pred_new.get(), b->nq.get(),
+                         ls, b->d_report, seq, level_bits);
       G500_CHECK_LAUNCH();
     } else {
       hipLaunchKernelGGL(k_bu_step, dim3(grid_for(ceil_div(words, kWave), kWavesPerBlock)),
                          dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(), words,
-                         g->row_off.get(), g->col.get(), g->new2orig.get(),
-                         b->pred.get(), cnt, level_bits);
+                         g->row_off.get(), g->col.get(), g->first.get(),
+                         g->new2orig.get(), b->pred_new.get(), ls, b->d_report, seq,
+                         level_bits);
       G500_CHECK_LAUNCH();
       std::swap(b->vis.p, b->vis_alt.p);
     }
     if (b->timing) G500_CHECK_HIP(hipEventRecord(lev1, s));
-    G500_CHECK_HIP(hipMemcpyAsync(b->h_counters, cnt, 4 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, s));
-    G500_CHECK_HIP(hipStreamSynchronize(s));
-    const int64_t nq_size = (int64_t)b->h_counters[0];
-    const int64_t nq_edges = td ? (int64_t)b->h_counters[1] : 0;
+    wait_report(b, seq, rep);
+    const int64_t nq_size = (int64_t)rep[0];
+    const int64_t nq_edges = td ? (int64_t)rep[1] : 0;
     g500_level_trace tr{};
     tr.level = level;
     tr.top_down = td ? 1 : 0;
     tr.nq_size = nq_size;
     tr.nq_edges = nq_edges;
-    if (b->timing) {
-      float ms = 0;
-      G500_CHECK_HIP(hipEventElapsedTime(&ms, lev0, lev1));
-      tr.gpu_ms = ms;
-      (td ? b->stats.td_ms : b->stats.bu_ms) += ms;
-    }
     b->trace.push_back(tr);
     if (nq_size == 0) break;
 
@@ -473,28 +584,39 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       est_edges = nq_edges;
     } else if (next_td) {
       // switching to top-down: frontier = bits new in this level
-      G500_CHECK_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), s));
+      const unsigned long long cseq = ++b->seq;
       hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)ceil_div(words, kT)), dim3(kT),
                          0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
-                         b->q.get(), cnt);
+                         b->q.get(), ls, b->d_report, cseq);
       G500_CHECK_LAUNCH();
-      G500_CHECK_HIP(hipMemcpyAsync(b->h_counters, cnt, 4 * sizeof(unsigned long long),
-                                    hipMemcpyDeviceToHost, s));
-      G500_CHECK_HIP(hipStreamSynchronize(s));
-      qlen = (int64_t)b->h_counters[2];
-      est_edges = (int64_t)b->h_counters[3];
+      wait_report(b, cseq, rep);
+      qlen = (int64_t)rep[0];
+      est_edges = (int64_t)rep[1];
       if (qlen != nq_size)
         fail(G500_ERR_STATE, "frontier conversion mismatch (%lld vs %lld)",
              (long long)qlen, (long long)nq_size);
     }
     prev_nq = nq_size;
   }
+  hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
+                     dim3(kT), 0, s, g->orig2new.get(), b->vis.get(), b->pred_new.get(),
+                     g->n, n_c, root, b->pred.get());
+  G500_CHECK_LAUNCH();
   G500_CHECK_HIP(hipEventRecord(b->ev_end, s));
   G500_CHECK_HIP(hipEventSynchronize(b->ev_end));
   float ms = 0;
   G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_begin, b->ev_end));
   b->stats.gpu_ms = ms;
   b->stats.nlevels = (int)b->trace.size();
+  if (b->timing) {
+    for (size_t i = 0; i < b->trace.size(); ++i) {
+      float lm = 0;
+      G500_CHECK_HIP(hipEventElapsedTime(&lm, b->level_events[2 * i],
+                                         b->level_events[2 * i + 1]));
+      b->trace[i].gpu_ms = lm;
+      (b->trace[i].top_down ? b->stats.td_ms : b->stats.bu_ms) += lm;
+    }
+  }
   for (const auto& tr : b->trace)
     if (tr.top_down) b->stats.td_edges += tr.nq_edges;
   b->have_result = true;
@@ -516,7 +638,7 @@ void bfs_levels(g500_bfs* b, uint16_t* out) {
 }  // namespace g500
 
 g500_bfs::~g500_bfs() {
-  if (h_counters) (void)hipHostFree(h_counters);
+  if (h_report) (void)hipHostFree(h_report);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
   for (auto e : level_events) (void)hipEventDestroy(e);

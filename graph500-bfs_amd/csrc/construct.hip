@@ -200,6 +200,17 @@ __global__ __launch_bounds__(kT) void k_dedup_write(
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
+// Copy of each row's first entry, stored densely so that the bottom-up sweep
+// reads it coalesced (the reference keeps the same "isolated first edge",
+// mpi/graph_constructor.hpp:1387-1557).
+__global__ __launch_bounds__(kT) void k_first_nbr(const int64_t* __restrict__ row_off,
+                                                 const uint32_t* __restrict__ col,
+                                                 int64_t n_c,
+                                                 uint32_t* __restrict__ first) {
+  const int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n_c) first[v] = col[row_off[v]];
+}
+
 int bitlen(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
 }  // namespace
@@ -303,6 +314,12 @@ void construct_graph(g500_graph* g, int corebfs) {
     hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(1), 0, s, g->row_off.get() + g->n_c,
                        g->m);
     G500_CHECK_LAUNCH();
+    g->first.alloc((size_t)std::max<int64_t>(g->n_c, 1));
+    if (g->n_c > 0) {
+      hipLaunchKernelGGL(k_first_nbr, dim3((unsigned)ceil_div(g->n_c, kT)), dim3(kT), 0, s,
+                         g->row_off.get(), g->col.get(), g->n_c, g->first.get());
+      G500_CHECK_LAUNCH();
+    }
     G500_CHECK_HIP(hipStreamSynchronize(s));
   }
   g->h_orig2new.clear();
@@ -321,7 +338,7 @@ void ensure_host_orig2new(g500_graph* g) {
 
 int64_t g500_graph::device_bytes() const {
   return (int64_t)(edges.n * sizeof(uint2) + row_off.n * sizeof(int64_t) +
-                   col.n * sizeof(uint32_t) +
+                   (col.n + first.n) * sizeof(uint32_t) +
                    (deg_new.n + new2orig.n + orig2new.n) * sizeof(uint32_t) +
                    tree_parent.n * sizeof(int64_t));
 }

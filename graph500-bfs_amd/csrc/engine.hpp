@@ -5,23 +5,40 @@
 
 #include "internal.hpp"
 
+// Device-side per-level counters; reset by the last workgroup of the kernel
+// that closes a level, after it has posted the totals to the host.
+struct LevelSync {
+  unsigned long long cnt[4];
+  unsigned int done;
+  unsigned int pad;
+};
+
+// Host-visible (pinned, coherent, mapped) level report.
+struct HostReport {
+  unsigned long long seq;
+  unsigned long long v[4];
+};
+
 struct g500_bfs {
   g500_graph* g = nullptr;
   hipStream_t stream = nullptr;
 
   int64_t words = 0;                // bitmap words covering [0, n_c)
   g500::DevBuf<int64_t> pred;       // n, by original ID (packed entries)
+  g500::DevBuf<int64_t> pred_new;   // n_c, the same entries by new ID
   g500::DevBuf<uint64_t> vis, vis_alt;  // visited bitmaps (new IDs)
   g500::DevBuf<uint32_t> q, nq;     // frontier queues (new IDs)
   g500::DevBuf<uint64_t> dsum;      // frontier degree prefix sums
-  g500::DevBuf<unsigned long long> counters;  // per-level counters
+  g500::DevBuf<LevelSync> lsync;   // per-level counters
+  HostReport* h_report = nullptr;  // pinned host memory, written by the GPU
+  HostReport* d_report = nullptr;  // its device address
+  unsigned long long seq = 0;      // last report sequence number issued
   g500::ScanWorkspace scan_ws;
 
   // validation scratch
   g500::DevBuf<unsigned long long> vcount;  // [G500_VERR_NUM + 2]
   g500::DevBuf<uint8_t> pred_valid;         // n
 
-  unsigned long long* h_counters = nullptr;  // pinned
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
   bool timing = false;
