@@ -133,12 +133,15 @@ int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
   });
 }
 
-int g500_graph_construct(g500_graph* g, int corebfs) {
+int g500_graph_construct(g500_graph* g, int flags) {
   return guard([&] {
     need_graph(g, false);
     use_device(g->device);
     if (g->constructed) g500::fail(G500_ERR_STATE, "graph already constructed");
-    if (corebfs) {
+    if (flags & ~(G500_CONSTRUCT_COREBFS | G500_CONSTRUCT_SYMMETRIC_INPUT))
+      g500::fail(G500_ERR_ARG, "unknown construct flags 0x%x", flags);
+    g->symmetric_input = (flags & G500_CONSTRUCT_SYMMETRIC_INPUT) != 0;
+    if (flags & G500_CONSTRUCT_COREBFS) {
       g500::corebfs_prune(g);
     } else {
       g500::construct_graph(g, 0);
@@ -252,6 +255,15 @@ int g500_find_roots(g500_graph* g, int nroots, int r1, int r2, int64_t* roots,
       roots[idx] = root;
     }
     *found = idx;
+  });
+}
+
+int g500_make_random_numbers(int64_t nvalues, uint64_t seed1, uint64_t seed2,
+                             int64_t position, double* out) {
+  return guard([&] {
+    if (nvalues < 0 || (nvalues && !out) || position < 0)
+      g500::fail(G500_ERR_ARG, "bad arguments");
+    g500::make_random_numbers_host(nvalues, seed1, seed2, position, out);
   });
 }
 

@@ -44,8 +44,9 @@ __device__ inline uint64_t wave_max(uint64_t v) {
 }
 
 // stats[0] |= endpoint IDs of non-loop edges, stats[1] += non-loop edges
+// (symmetric input: each directed edge counts once, toward its source)
 __global__ __launch_bounds__(kT) void k_degree(const uint2* __restrict__ e,
-                                              int64_t ne,
+                                              int64_t ne, int symmetric,
                                               uint32_t* __restrict__ deg,
                                               unsigned long long* stats) {
   uint64_t orv = 0, cnt = 0;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(kT) void k_degree(const uint2* __restrict__ e,
     const uint2 x = e[i];
     if (x.x == x.y) continue;
     atomicAdd(&deg[x.x], 1u);
-    atomicAdd(&deg[x.y], 1u);
+    if (!symmetric) atomicAdd(&deg[x.y], 1u);
     orv |= (uint64_t)(x.x | x.y);
     ++cnt;
   }
@@ -113,8 +114,10 @@ __global__ __launch_bounds__(kT) void k_relabel(const uint64_t* __restrict__ key
   deg_new[i] = deg[o];
 }
 
+// Two keys per input edge ((a,b) and (b,a)), or one when the input is
+// already symmetric; self loops become the sentinel (sorts last).
 __global__ __launch_bounds__(kT) void k_sym_keys(const uint2* __restrict__ e,
-                                                int64_t ne,
+                                                int64_t ne, int symmetric,
                                                 const uint32_t* __restrict__ o2n,
                                                 int lg, uint64_t sentinel,
                                                 uint64_t* __restrict__ keys) {
@@ -127,8 +130,12 @@ __global__ __launch_bounds__(kT) void k_sym_keys(const uint2* __restrict__ e,
     k0 = (a << lg) | b;
     k1 = (b << lg) | a;
   }
-  keys[2 * i] = k0;
-  keys[2 * i + 1] = k1;
+  if (symmetric) {
+    keys[i] = k0;
+  } else {
+    keys[2 * i] = k0;
+    keys[2 * i + 1] = k1;
+  }
 }
 
 constexpr int kDedupItems = 16;
@@ -229,7 +236,7 @@ void construct_graph(g500_graph* g, int corebfs) {
   if (ne > 0) {
     const int64_t blocks = std::min<int64_t>(ceil_div(ne, kT), 8192);
     hipLaunchKernelGGL(k_degree, dim3((unsigned)blocks), dim3(kT), 0, s,
-                       g->edges.get(), ne, deg.get(), stats.get());
+                       g->edges.get(), ne, (int)g->symmetric_input, deg.get(), stats.get());
     G500_CHECK_LAUNCH();
   }
   {
@@ -251,7 +258,7 @@ void construct_graph(g500_graph* g, int corebfs) {
   g->n_c = (int64_t)h[3];
   g->max_used = (int64_t)h[4];
   g->max_deg = (int64_t)maxdeg;
-  g->m_sym = 2 * nonloop;
+  g->m_sym = g->symmetric_input ? nonloop : 2 * nonloop;
 
   // ---- relabel by (degree desc, id asc)
   g->new2orig.alloc((size_t)n);
@@ -278,15 +285,15 @@ void construct_graph(g500_graph* g, int corebfs) {
 
   // ---- symmetric (new_src, new_dst) keys, sorted, then deduplicated
   {
-    const int64_t nk = 2 * ne;
+    const int64_t nk = g->symmetric_input ? ne : 2 * ne;
     DevBuf<uint64_t> ek, ek_alt;
     ek.alloc((size_t)nk);
     ek_alt.alloc((size_t)nk);
     const uint64_t sentinel = ((uint64_t)(n - 1) << lg) | (uint64_t)(n - 1);
     if (ne > 0) {
       hipLaunchKernelGGL(k_sym_keys, dim3((unsigned)ceil_div(ne, kT)), dim3(kT), 0,
-                         s, g->edges.get(), ne, g->orig2new.get(), lg, sentinel,
-                         ek.get());
+                         s, g->edges.get(), ne, (int)g->symmetric_input,
+                         g->orig2new.get(), lg, sentinel, ek.get());
       G500_CHECK_LAUNCH();
     }
     uint64_t* sorted = radix_sort_u64(ek.get(), ek_alt.get(), nk, 2 * lg,

@@ -16,6 +16,7 @@ struct g500_graph {
   g500::DevBuf<uint2> edges;
   int64_t nedges = 0;
   int id_bits = 0;  // every input ID < 2^id_bits
+  bool symmetric_input = false;  // edges already hold both directions
 
   // construction results (all CSR arrays indexed by relabelled "new" ID)
   bool constructed = false;

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "libg500bfs_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "g500bfs.h")
 
 AT_NUM = 9
+CONSTRUCT_COREBFS = 1
+CONSTRUCT_SYMMETRIC_INPUT = 2
 VERR_NUM = 8
 VERR_NAMES = ["range", "root", "self_parent", "depth_map", "edge_tree",
               "edge_gap", "pred_depth", "no_parent_edge"]
@@ -65,6 +67,7 @@ def load():
         "g500_graph_copy_csr": [vp, i64p, u32p, i64p, u32p, u32p],
         "g500_find_roots": [vp, C.c_int, C.c_int, C.c_int, i64p, C.POINTER(C.c_int)],
         "g500_graph_free": [vp],
+        "g500_make_random_numbers": [i64, C.c_uint64, C.c_uint64, i64, C.POINTER(C.c_double)],
         "g500_bfs_create": [vp, C.POINTER(vp)],
         "g500_bfs_run": [vp, i64, C.c_int, C.c_double, C.c_double, i64p],
         "g500_bfs_get_pred": [vp, i64p],
@@ -82,6 +85,12 @@ def load():
     L.g500_last_error.restype = C.c_char_p
     _lib = L
     return L
+
+
+def make_random_numbers(n, seed1, seed2, position):
+    out = np.empty(n, np.float64)
+    _check(load().g500_make_random_numbers(n, seed1, seed2, position, _p(out, C.c_double)))
+    return out
 
 
 def _check(rc):
@@ -115,8 +124,9 @@ class Graph:
                                             len(v0), C.byref(h)))
         return cls(h)
 
-    def construct(self, corebfs=False):
-        _check(load().g500_graph_construct(self.h, int(corebfs)))
+    def construct(self, corebfs=False, symmetric_input=False):
+        flags = (CONSTRUCT_COREBFS if corebfs else 0) | (CONSTRUCT_SYMMETRIC_INPUT if symmetric_input else 0)
+        _check(load().g500_graph_construct(self.h, flags))
         self._info = None
         return self
 

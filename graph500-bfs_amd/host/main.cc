@@ -1,0 +1,267 @@
+// main.cc -- Graph500 BFS driver on one MI355X.
+//
+// Same command line, flow and result block as the reference driver
+// (mpi/main.cc:393-870): generate -> construct -> roots (redraw rule) ->
+// optional alpha/beta tuning (-A, -S) -> timed run_bfs per root -> get_pred ->
+// validate -> result block.  The graph is generated and built on the GPU, the
+// BFS runs there, and validation runs there against the resident edge list.
+#include <getopt.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+#include "autotune.hpp"
+#include "bfs_mi355x.hpp"
+#include "report.hpp"
+
+using g500host::AutoTuner;
+using g500host::BfsOnMI355X;
+
+namespace {
+
+constexpr int kDefaultEdgeFactor = 16;  // mpi/parameters.h:96
+constexpr double kDefaultAlpha = 10.0;  // mpi/parameters.h:97
+constexpr double kDefaultBeta = 14.0;   // mpi/parameters.h:98
+constexpr int kRealBfsRoots = 64;       // mpi/parameters.h:88
+constexpr int kTestBfsRoots = 16;       // mpi/parameters.h:89
+constexpr int kUserSeed1 = 2, kUserSeed2 = 3;  // mpi/parameters.h:204-205
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+const double g_t0 = now();
+
+__attribute__((format(printf, 1, 2))) void say(const char* fmt, ...) {
+  std::printf("[r:0,%f] ", now() - g_t0);
+  va_list ap;
+  va_start(ap, fmt);
+  std::vprintf(fmt, ap);
+  va_end(ap);
+  std::printf("\n");
+  std::fflush(stdout);
+}
+
+[[noreturn]] void usage(const char* argv0) {
+  std::fprintf(stderr,
+               "\nUsage: %s SCALE [OPTION]...\n\n"
+               "Options:\n"
+               "  -e <edge factor>    Set an edge factor\n"
+               "  -a <alpha>          Set alpha\n"
+               "  -b <beta>           Set beta\n"
+               "  -r <root_seed>      Set the seed to find roots\n"
+               "  -n <count>          Set the number of search keys\n"
+               "  -s <count>          Skip the specified number of search keys\n"
+               "  -v <level>          Validation level\n"
+               "  -A                  Enable auto-tuning of alpha and beta\n"
+               "  -S                  Enable root seed searching\n"
+               "  -C                  Enable CoreBFS\n"
+               "  -P                  Enable pre-execution\n"
+               "  -R                  Set options at once as specified in the Graph500 spec.\n"
+               "  -d <device>         GPU to use (default 0)\n\n"
+               "Validation levels:\n"
+               "  0: No validation\n"
+               "  1: Validate the result of the first search key only\n"
+               "  2: Validate all results (required by Graph500 specification)\n",
+               argv0);
+  std::exit(1);
+}
+
+[[noreturn]] void die(const char* msg) {
+  std::fprintf(stderr, "%s", msg);
+  std::exit(1);
+}
+
+// AutoTuner runner over the GPU engine.
+struct EngineRunner {
+  BfsOnMI355X* b;
+  int64_t* pred;
+  int ef;
+  double run(int64_t root, double alpha, double beta, int64_t* at) {
+    const double t = now();
+    b->run_bfs(root, pred, ef, alpha, beta, at);
+    return now() - t;
+  }
+  int find_roots(int64_t* roots, int n, int r1, int r2) {
+    return b->find_roots(roots, n, r1, r2);
+  }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc <= 1 || std::atoi(argv[1]) <= 0) usage(argv[0]);
+  const int scale = std::atoi(argv[1]);
+  int ef = kDefaultEdgeFactor, root_seed = 0, num_roots = kTestBfsRoots, root_start = 0;
+  int vlevel = 2, tune_mode = 0, device = 0;
+  double alpha = kDefaultAlpha, beta = kDefaultBeta;
+  bool corebfs = false, pre_exec = false, real = false;
+  int c;
+  optind = 2;
+  while ((c = getopt(argc, argv, "e:a:b:r:n:s:v:ASCPRd:")) != -1) {
+    switch (c) {
+      case 'e': ef = std::atoi(optarg); if (ef <= 0) die("-e value > 0\n"); break;
+      case 'a': alpha = std::atof(optarg); if (alpha <= 0) die("-a value > 0\n"); break;
+      case 'b': beta = std::atof(optarg); if (beta <= 0) die("-b value > 0\n"); break;
+      case 'r': root_seed = std::atoi(optarg); break;
+      case 'n': num_roots = std::atoi(optarg); if (num_roots <= 0) die("-n value > 0\n"); break;
+      case 's': root_start = std::atoi(optarg); if (root_start <= 0) die("-s value > 0\n"); break;
+      case 'v':
+        vlevel = std::atoi(optarg);
+        if (vlevel < 0 || vlevel > 2) die("-v value >= 0 && value <= 2\n");
+        break;
+      case 'A': tune_mode = 1; break;
+      case 'S': tune_mode = 2; break;
+      case 'C': corebfs = true; break;
+      case 'P': pre_exec = true; break;
+      case 'R': real = true; break;
+      case 'd': device = std::atoi(optarg); break;
+      default: usage(argv[0]);
+    }
+  }
+  if (real) {
+    num_roots = kRealBfsRoots;
+    vlevel = 2;
+    pre_exec = true;
+  }
+  if (root_start != 0 && tune_mode) die("-s and -A are not available at the same time.\n");
+  if (root_start >= num_roots) die("-s value must be below -n\n");
+
+  try {
+    say("Graph500 Benchmark: SCALE: %d, edgefactor: %d ", scale, ef);
+    say("Running Binary: %s", argv[0]);
+    BfsOnMI355X bench(device);
+    BfsOnMI355X::printInformation(vlevel, pre_exec, real);
+
+    say("Graph generation");
+    double generation_time = now();
+    bench.generate(scale, ef, kUserSeed1, kUserSeed2);
+    generation_time = now() - generation_time;
+
+    say("Graph construction");
+    double construction_time = now();
+    bench.construct_generated(corebfs);
+    construction_time = now() - construction_time;
+    say("# of edges is reduced. Total -> %" PRId64 ", vertices with edges %" PRId64,
+        bench.graph_.num_global_edges_, bench.graph_.num_global_verts_);
+
+    const int64_t nlocalverts = bench.graph_.pred_size();
+    std::vector<int64_t> pred((size_t)nlocalverts);
+    std::vector<int64_t> roots((size_t)num_roots);
+    bench.prepare_bfs(vlevel, pre_exec, real, ef, alpha, beta, pred.data());
+
+    EngineRunner runner{&bench, pred.data(), ef};
+    AutoTuner<EngineRunner> tuner(runner, scale, ef);
+    std::vector<int64_t[g500host::kAtNum]> at((size_t)num_roots);
+    std::vector<double> perf((size_t)num_roots);
+    if (tune_mode == 0) {
+      tuner.find_roots_in_large_graph(root_start, num_roots, roots.data(), at.data(), alpha,
+                                      beta, perf.data(), root_seed);
+    } else {
+      if (scale > 43) die("Auto-tuning option cannot be supported SCALE > 43\n");
+      const double t_tune = now();
+      int find_root_seed = root_seed;
+      if (tune_mode == 2) {
+        const int trials = 100;
+        double max_teps = 0.0, sa = alpha, sb = beta;
+        for (int i = 0; i < trials; ++i) {
+          double rndd = 0.0;
+          if (i > 0)
+            g500host::check(g500_make_random_numbers(1, kUserSeed1, kUserSeed2, i, &rndd),
+                            "make_random_numbers");
+          const int seed = (int)(rndd * (1 << 24));
+          say("Trial seed %d/%d Seed: %d", i + 1, trials, seed);
+          const double teps = tuner.auto_tuning(root_start, num_roots, seed, roots.data(),
+                                                &sa, &sb);
+          if (teps > max_teps) {
+            find_root_seed = seed;
+            max_teps = teps;
+          }
+        }
+        say("Found seed %d", find_root_seed);
+      }
+      tuner.auto_tuning(root_start, num_roots, find_root_seed, roots.data(), &alpha, &beta);
+      say("Elapsed Time for auto tuning = %f sec.", now() - t_tune);
+    }
+
+    std::vector<double> bfs_times((size_t)num_roots), validate_times((size_t)num_roots),
+        edge_counts((size_t)num_roots);
+    std::vector<int64_t[g500host::kAtNum]> at_run((size_t)num_roots);
+    for (auto& row : at_run)
+      for (auto& x : row) x = g500host::kAtNotDefined;
+    bool result_ok = true;
+    double td_total = 0, bu_total = 0;
+    const int64_t golden = (ef == 16) ? g500host::pf_nedge(scale) : -1;
+    for (int i = root_start; i < num_roots; ++i) {
+      say("========== Running BFS %02d ==========", i);
+      bfs_times[i] = now();
+      bench.run_bfs(roots[i], pred.data(), ef, alpha, beta, at_run[i]);
+      bfs_times[i] = now() - bfs_times[i];
+      say("Time for BFS %02d is %f (%.*e)", i, bfs_times[i],
+          std::numeric_limits<double>::max_digits10, bfs_times[i]);
+      g500_run_stats st;
+      if (bench.stats(&st)) {
+        td_total += st.td_ms * 1e-3;
+        bu_total += st.bu_ms * 1e-3;
+      }
+      bench.get_pred(pred.data());
+      validate_times[i] = now();
+      int64_t nedge = 0;
+      if (vlevel == 2 || (vlevel == 1 && i == root_start)) {
+        result_ok = bench.validate(roots[i], &nedge);
+      } else {
+        nedge = golden;
+        if (vlevel == 0) {
+          say("Sleep 10s");
+          std::this_thread::sleep_for(std::chrono::seconds(10));
+        }
+      }
+      validate_times[i] = now() - validate_times[i];
+      edge_counts[i] = (double)nedge;
+      say("Validate time for BFS %02d is %f", i, validate_times[i]);
+      say("Number of traversed edges is %" PRId64, nedge);
+      say("TEPS for BFS %02d is %g", i, nedge / bfs_times[i]);
+      if (!result_ok) break;
+    }
+    bench.end_bfs();
+
+    std::printf("============= Result ==============\n");
+    std::printf("SCALE:                          %d\n", scale);
+    std::printf("edgefactor:                     %d\n", ef);
+    std::printf("alpha:                          %f\n", alpha);
+    std::printf("beta:                           %f\n", beta);
+    std::printf("NBFS:                           %d\n", num_roots);
+    std::printf("graph_generation:               %g\n", generation_time);
+    std::printf("num_mpi_processes:              %d\n", 1);
+    std::printf("construction_time:              %g\n", construction_time);
+    std::printf("redistribution_time:            %g\n", 0.0);
+    std::vector<double> bt(bfs_times.begin() + root_start, bfs_times.end()),
+        vt(validate_times.begin() + root_start, validate_times.end()),
+        ec(edge_counts.begin() + root_start, edge_counts.end());
+    g500host::print_bfs_result(bt, vt, ec, result_ok);
+    double total = 0;
+    for (double t : bt) total += t;
+    std::printf("---\n");
+    std::printf("CATEGORY :                 :   TOTAL\n");
+    std::printf("TOTAL                      : %6.5f\n", total);
+    if (td_total + bu_total > 0) {
+      std::printf(" - TOP_DOWN                : %6.5f (%6.5f%%)\n", td_total, 100 * td_total / total);
+      std::printf(" - BOTTOM_UP               : %6.5f (%6.5f%%)\n", bu_total, 100 * bu_total / total);
+      std::printf(" - OTHER                   : %6.5f\n", total - td_total - bu_total);
+    }
+    std::fflush(stdout);
+    return result_ok ? 0 : 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 2;
+  }
+}

@@ -98,11 +98,20 @@ int g500_graph_generate(int device, int scale, int edgefactor, uint64_t seed1,
 int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
                           int64_t nedges, g500_graph** out);
 
+/* flags of g500_graph_construct */
+enum {
+  G500_CONSTRUCT_COREBFS = 1,        /* prune trees first (-C) */
+  G500_CONSTRUCT_SYMMETRIC_INPUT = 2 /* edges already hold both directions */
+};
+
 /* Symmetrize, relabel by degree and build the CSR on the GPU; with
- * corebfs != 0 also prune trees (CoreBFS) first.  Replaces
+ * G500_CONSTRUCT_COREBFS also prune trees (CoreBFS) first.  Replaces
  * make_symmetry_edge_list (mpi/benchmark_helper.hpp:389-497) +
- * BfsBase::construct (mpi/bfs.hpp:116-139). */
-int g500_graph_construct(g500_graph* g, int corebfs);
+ * BfsBase::construct (mpi/bfs.hpp:116-139).  With
+ * G500_CONSTRUCT_SYMMETRIC_INPUT the edge list is taken as the output of
+ * make_symmetry_edge_list (what BfsBase::construct receives): each directed
+ * edge counts once toward its source's degree. */
+int g500_graph_construct(g500_graph* g, int flags);
 
 int g500_graph_get_info(const g500_graph* g, g500_graph_info* info);
 
@@ -124,6 +133,12 @@ int g500_graph_copy_csr(const g500_graph* g, int64_t* row_off, uint32_t* col,
  * seeds (seed1 + r1, seed2 + r2); returns the count in *found. */
 int g500_find_roots(g500_graph* g, int nroots, int r1, int r2, int64_t* roots,
                     int* found);
+
+/* make_random_numbers (generator/graph_generator.hpp:503-520): nvalues
+ * doubles from the MRG stream of (seed1, seed2) starting at `position`
+ * (the driver's -S root-seed search uses it). */
+int g500_make_random_numbers(int64_t nvalues, uint64_t seed1, uint64_t seed2,
+                             int64_t position, double* out);
 
 int g500_graph_free(g500_graph* g);
 

@@ -1,0 +1,99 @@
+"""GPU tests of the drop-in layer: the C++ driver binary (reference CLI and
+result block), BfsOnMI355X::construct(EdgeList*) fed the symmetric list the
+reference passes it, and the symmetric-input construction flag."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "graph500-bfs_amd")
+RUNNABLE = os.path.join(PKG, "bin", "g500_runnable")
+
+
+def _run(args, timeout=240):
+    print(f"\n[driver] {' '.join(args)}", flush=True)
+    p = subprocess.run([RUNNABLE, *args], capture_output=True, text=True, timeout=timeout)
+    return p.returncode, p.stdout + p.stderr
+
+
+def _block(out):
+    body = out.split("============= Result ==============")[1]
+    return {k: float(v) for k, v in re.findall(r"^([a-z_A-Z]+):\s+(\S+)$", body, re.M)}
+
+
+def test_driver_scale16_result_block(golden):
+    rc, out = _run(["16", "-n", "16"])
+    assert rc == 0, out[-3000:]
+    b = _block(out)
+    assert b["SCALE"] == 16 and b["NBFS"] == 16
+    assert b["min_nedge"] == b["max_nedge"] == golden["pf_nedge"][16]
+    assert b["harmonic_mean_TEPS"] > 0
+    roots = [int(x) for x in re.findall(r"Running BFS (\d+)", out)]
+    assert roots == list(range(16))
+
+
+def test_driver_validation_levels_and_skip():
+    rc, out = _run(["14", "-n", "6", "-s", "2", "-v", "1"])
+    assert rc == 0, out[-3000:]
+    assert _block(out)["NBFS"] == 6
+    assert "Running BFS 00" not in out and "Running BFS 02" in out
+    rc, out = _run(["14", "-s", "3", "-A"])
+    assert rc != 0 and "-s and -A" in out
+
+
+def test_driver_autotune():
+    rc, out = _run(["16", "-n", "8", "-A"])
+    assert rc == 0, out[-3000:]
+    assert "START AUTO TUNING FOR ALPHA" in out and "START AUTO TUNING FOR BETA" in out
+    assert "Estimated Performance" in out
+    assert _block(out)["harmonic_mean_TEPS"] > 0
+
+
+def _sym(v0, v1):
+    keep = v0 != v1
+    return (np.concatenate([v0[keep], v1[keep]]), np.concatenate([v1[keep], v0[keep]]))
+
+
+def test_symmetric_input_flag_matches_raw_list():
+    import g500bfs
+    v0, v1 = O.generate_edges(12)
+    s0, s1 = _sym(v0, v1)
+    ga = g500bfs.Graph.from_edges(v0, v1).construct()
+    gb = g500bfs.Graph.from_edges(s0, s1).construct(symmetric_input=True)
+    for k in ("n", "n_c", "m", "m_sym", "max_degree"):
+        assert getattr(ga.info, k) == getattr(gb.info, k), k
+    a, b = ga.csr(), gb.csr()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    ba, bb = g500bfs.Bfs(ga), g500bfs.Bfs(gb)
+    for r in ga.find_roots(4):
+        assert ba.run(r).tolist() == bb.run(r).tolist()
+        assert np.array_equal(ba.levels(), bb.levels())
+
+
+def test_dropin_construct_from_edge_list(tmp_path, golden):
+    exe = str(tmp_path / "dropin")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                    "-o", exe, os.path.join(ROOT, "tests", "cpp", "dropin_main.cc"),
+                    "-L" + PKG, "-lg500bfs_hip", "-Wl,-rpath," + PKG], check=True)
+    v0, v1 = O.generate_edges(14)
+    s0, s1 = _sym(v0, v1)
+    np.stack([s0, s1], 1).astype(np.int64).tofile(tmp_path / "sym.bin")
+    og = O.Graph(v0, v1)
+    root = int(og.find_roots(1)[0])
+    p = subprocess.run([exe, str(tmp_path / "sym.bin"), str(root), str(tmp_path / "pred.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    pred = np.fromfile(tmp_path / "pred.bin", np.int64)
+    ok, te, errs = og.validate(root, pred)  # the reference validates with the raw list
+    assert ok, errs
+    assert te == golden["pf_nedge"][14]
+    lev = og.bfs(root)
+    d = (pred.astype(np.uint64) >> np.uint64(48)).astype(np.int64)
+    assert np.array_equal(np.where(d == 0xFFFF, -1, d), lev)
