@@ -74,7 +74,7 @@ bool has_edge_host(g500_graph* g, int64_t v) {
   if (v < 0 || v >= g->n) return false;
   g500::ensure_host_orig2new(g);
   if ((int64_t)g->h_orig2new[(size_t)v] < g->n_c) return true;
-  return g->corebfs && !g->h_tree_parent.empty() && g->h_tree_parent[(size_t)v] >= 0;
+  return g500::has_parent_host(g, v);
 }
 
 }  // namespace
@@ -166,6 +166,23 @@ int g500_graph_get_info(const g500_graph* g, g500_graph_info* info) {
     info->core_vertices = g->core_vertices;
     info->tree_vertices = g->tree_vertices;
     info->device_bytes = g->device_bytes();
+    info->removed_parents = g->removed_parents;
+    info->hd_root = g->corebfs ? g->hd_root : -1;
+  });
+}
+
+int g500_graph_copy_forest(const g500_graph* g, int64_t* tree_parent, uint8_t* has_parent) {
+  return guard([&] {
+    need_graph(g, true);
+    if (!g->corebfs) g500::fail(G500_ERR_STATE, "graph was not built with CoreBFS");
+    use_device(g->device);
+    std::vector<uint32_t> tp((size_t)g->n);
+    G500_CHECK_HIP(hipMemcpy(tp.data(), g->tree_parent.get(), g->n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+    for (int64_t o = 0; o < g->n; ++o) {
+      if (tree_parent) tree_parent[o] = tp[(size_t)o] == g500::kNoParent ? -1 : (int64_t)tp[(size_t)o];
+      if (has_parent) has_parent[o] = g500::has_parent_host(g, o) ? 1 : 0;
+    }
   });
 }
 

@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -384,20 +386,42 @@ __global__ __launch_bounds__(kT) void k_bitmap_to_queue(
 }
 
 // pred[o] for every original ID: the level-written entry of its new ID if
-// that vertex was reached, the root's own entry, else -1.
+// that vertex was reached, the root's own entry, else -1.  With CoreBFS a
+// forest vertex not reached through the core gets its forest parent with
+// depth 0xFFFF (write_tree_parents, mpi/corebfs_adaptor.hpp:285-301).
 __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__ o2n,
                                                    const uint64_t* __restrict__ vis,
                                                    const int64_t* __restrict__ pred_new,
                                                    int64_t n, int64_t n_c, int64_t root,
+                                                   const uint32_t* __restrict__ tree_parent,
+                                                   const uint64_t* __restrict__ has_parent,
                                                    int64_t* __restrict__ pred) {
   for (int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x; o < n;
        o += (int64_t)gridDim.x * kT) {
     const uint32_t v = o2n[o];
     int64_t e = -1;
-    if ((int64_t)v < n_c && ((vis[v >> 6] >> (v & 63)) & 1ull)) e = pred_new[v];
+    if ((int64_t)v < n_c && ((vis[v >> 6] >> (v & 63)) & 1ull)) {
+      e = pred_new[v];
+    } else if (has_parent && ((has_parent[o >> 6] >> (o & 63)) & 1ull)) {
+      const uint32_t p = tree_parent[o];
+      if (p != kNoParent) e = (int64_t)((uint64_t)p | (0xFFFFull << 48));
+    }
     if (o == root) e = root;
     pred[o] = e;
   }
+}
+
+// The walked path from the root to the core, with real depths.
+constexpr int kMaxPathArgs = 32;
+struct PathArgs {
+  int count;
+  uint32_t v[kMaxPathArgs];
+  int64_t e[kMaxPathArgs];
+};
+
+__global__ void k_write_path(PathArgs pa, int64_t* __restrict__ pred) {
+  const int i = threadIdx.x;
+  if (i < pa.count) pred[pa.v[i]] = pa.e[i];
 }
 
 __global__ __launch_bounds__(kT) void k_levels(const int64_t* __restrict__ pred,
@@ -433,6 +457,17 @@ void bfs_init(g500_bfs* b) {
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
   ensure_host_orig2new(g);
+  if (g->corebfs && !g->core_reset_done) {
+    // corebfs_index::reset_unreachable: one BFS over the core from its
+    // highest-degree vertex decides which forest parents survive
+    if (g->hd_root >= 0) {
+      bfs_run(b, g->hd_root, 16, 10.0, 14.0, nullptr);
+      corebfs_reset_unreachable(g, b->vis.get(), b->stream);
+    } else {
+      corebfs_reset_unreachable(g, nullptr, b->stream);
+    }
+    b->have_result = false;
+  }
 }
 
 namespace {
@@ -475,7 +510,30 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     at = at_local;
   }
   const int64_t n_c = g->n_c, words = b->words;
-  const int64_t root_new = g->h_orig2new.empty() ? -1 : (int64_t)g->h_orig2new[root];
+  // CoreBFS: walk up the forest to the core (bfs_tree_with_callback,
+  // external/indexed_bfs/bfs/corebfs.hpp:125-157); the path vertices get
+  // parents pointing back toward the root and their true depths
+  std::vector<std::pair<uint32_t, int64_t>> path;
+  int64_t bfs_root = root;
+  int root_level = 0;
+  if (g->corebfs) {
+    int64_t u = root, p = root;
+    for (int lv = 0;; ++lv) {
+      if (lv > (1 << 20)) fail(G500_ERR_STATE, "forest walk from %lld does not end", (long long)root);
+      path.emplace_back((uint32_t)p, (int64_t)(((uint64_t)lv << 48) | (uint64_t)u));
+      u = p;
+      p = (has_parent_host(g, u) && !g->h_tree_parent.empty() &&
+           g->h_tree_parent[(size_t)u] != kNoParent)
+              ? (int64_t)g->h_tree_parent[(size_t)u]
+              : -1;
+      if (p < 0) {
+        bfs_root = u;
+        root_level = lv;
+        break;
+      }
+    }
+  }
+  const int64_t root_new = g->h_orig2new.empty() ? -1 : (int64_t)g->h_orig2new[bfs_root];
   const bool root_in = root_new >= 0 && root_new < n_c;
 
   b->trace.clear();
@@ -486,7 +544,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1);
   G500_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, b->pred_new.get(),
-                     root /* parent = root, depth 0 */, b->q.get(),
+                     (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get(),
                      root_in ? root_new : (int64_t)-1, n_c);
   G500_CHECK_LAUNCH();
 
@@ -496,7 +554,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   // level-loop state (mpi/bfs.hpp:3264-3279)
   bool next_td = true, growing = true;
   int64_t prev_nq = 0;
-  int level = 0;
+  int level = root_level;
   const int64_t alpha_thr = (int64_t)((double)g->m / alpha);
   const int64_t beta_thr = (int64_t)((double)n_c / (beta * edgefactor * 2.0));
   LevelSync* ls = b->lsync.get();
@@ -600,8 +658,19 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   }
   hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
                      dim3(kT), 0, s, g->orig2new.get(), b->vis.get(), b->pred_new.get(),
-                     g->n, n_c, root, b->pred.get());
+                     g->n, n_c, root, g->corebfs ? g->tree_parent.get() : nullptr,
+                     g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
   G500_CHECK_LAUNCH();
+  for (size_t i = 0; i < path.size(); i += kMaxPathArgs) {
+    PathArgs pa;
+    pa.count = (int)std::min<size_t>(kMaxPathArgs, path.size() - i);
+    for (int j = 0; j < pa.count; ++j) {
+      pa.v[j] = path[i + j].first;
+      pa.e[j] = path[i + j].second;
+    }
+    hipLaunchKernelGGL(k_write_path, dim3(1), dim3(kMaxPathArgs), 0, s, pa, b->pred.get());
+    G500_CHECK_LAUNCH();
+  }
   G500_CHECK_HIP(hipEventRecord(b->ev_end, s));
   G500_CHECK_HIP(hipEventSynchronize(b->ev_end));
   float ms = 0;
@@ -619,6 +688,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   }
   for (const auto& tr : b->trace)
     if (tr.top_down) b->stats.td_edges += tr.nq_edges;
+  b->stats.root_level = root_level;
   b->have_result = true;
   b->last_root = root;
 }

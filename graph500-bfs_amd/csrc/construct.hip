@@ -222,9 +222,9 @@ int bitlen(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
 }  // namespace
 
-void construct_graph(g500_graph* g, int corebfs) {
+void build_csr(g500_graph* g, const uint2* edges, int64_t ne, bool symmetric,
+               int force_log_n) {
   hipStream_t s = g->stream;
-  const int64_t ne = g->nedges;
   const int64_t nv = (int64_t)1 << g->id_bits;  // covers every input ID
 
   DevBuf<unsigned long long> stats;
@@ -236,7 +236,7 @@ void construct_graph(g500_graph* g, int corebfs) {
   if (ne > 0) {
     const int64_t blocks = std::min<int64_t>(ceil_div(ne, kT), 8192);
     hipLaunchKernelGGL(k_degree, dim3((unsigned)blocks), dim3(kT), 0, s,
-                       g->edges.get(), ne, (int)g->symmetric_input, deg.get(), stats.get());
+                       edges, ne, (int)symmetric, deg.get(), stats.get());
     G500_CHECK_LAUNCH();
   }
   {
@@ -251,14 +251,14 @@ void construct_graph(g500_graph* g, int corebfs) {
   const uint64_t ormask = h[0];
   const int64_t nonloop = (int64_t)h[1];
   const uint64_t maxdeg = h[2];
-  const int lg = bitlen(ormask);  // msb + 1, 0 for an edgeless graph
+  const int lg = force_log_n >= 0 ? force_log_n : bitlen(ormask);  // msb + 1
   const int64_t n = (int64_t)1 << lg;
   g->log_n = lg;
   g->n = n;
   g->n_c = (int64_t)h[3];
   g->max_used = (int64_t)h[4];
   g->max_deg = (int64_t)maxdeg;
-  g->m_sym = g->symmetric_input ? nonloop : 2 * nonloop;
+  g->m_sym = symmetric ? nonloop : 2 * nonloop;
 
   // ---- relabel by (degree desc, id asc)
   g->new2orig.alloc((size_t)n);
@@ -285,15 +285,15 @@ void construct_graph(g500_graph* g, int corebfs) {
 
   // ---- symmetric (new_src, new_dst) keys, sorted, then deduplicated
   {
-    const int64_t nk = g->symmetric_input ? ne : 2 * ne;
+    const int64_t nk = symmetric ? ne : 2 * ne;
     DevBuf<uint64_t> ek, ek_alt;
     ek.alloc((size_t)nk);
     ek_alt.alloc((size_t)nk);
     const uint64_t sentinel = ((uint64_t)(n - 1) << lg) | (uint64_t)(n - 1);
     if (ne > 0) {
       hipLaunchKernelGGL(k_sym_keys, dim3((unsigned)ceil_div(ne, kT)), dim3(kT), 0,
-                         s, g->edges.get(), ne, (int)g->symmetric_input,
-                         g->orig2new.get(), lg, sentinel, ek.get());
+                         s, edges, ne, (int)symmetric, g->orig2new.get(), lg, sentinel,
+                         ek.get());
       G500_CHECK_LAUNCH();
     }
     uint64_t* sorted = radix_sort_u64(ek.get(), ek_alt.get(), nk, 2 * lg,
@@ -330,6 +330,10 @@ void construct_graph(g500_graph* g, int corebfs) {
     G500_CHECK_HIP(hipStreamSynchronize(s));
   }
   g->h_orig2new.clear();
+}
+
+void construct_graph(g500_graph* g, int corebfs) {
+  build_csr(g, g->edges.get(), g->nedges, g->symmetric_input, -1);
   g->corebfs = corebfs;
   g->constructed = true;
 }
@@ -347,5 +351,5 @@ int64_t g500_graph::device_bytes() const {
   return (int64_t)(edges.n * sizeof(uint2) + row_off.n * sizeof(int64_t) +
                    (col.n + first.n) * sizeof(uint32_t) +
                    (deg_new.n + new2orig.n + orig2new.n) * sizeof(uint32_t) +
-                   tree_parent.n * sizeof(int64_t));
+                   tree_parent.n * sizeof(uint32_t) + has_parent.n * sizeof(uint64_t));
 }

@@ -38,11 +38,16 @@ struct g500_graph {
   // host copy of orig2new (lazily made for has_edge / root selection)
   std::vector<uint32_t> h_orig2new;
 
-  // CoreBFS: pruned-forest parents (original IDs, -1 = in core / none),
-  // valid when corebfs != 0
-  g500::DevBuf<int64_t> tree_parent;  // n
-  std::vector<int64_t> h_tree_parent;
-  int64_t core_vertices = 0, tree_vertices = 0;
+  // CoreBFS (corebfs != 0): forest parents by original ID (kNoParent = none
+  // or reset as unreachable) and the bitmap of vertices that were given a
+  // parent while pruning (has_edge counts them, mpi/bfs.hpp:167-170)
+  g500::DevBuf<uint32_t> tree_parent;     // n
+  g500::DevBuf<uint64_t> has_parent;      // n/64 words
+  std::vector<uint32_t> h_tree_parent;    // host copy for the per-root walk
+  std::vector<uint64_t> h_has_parent;
+  bool core_reset_done = false;           // reset_unreachable applied
+  int64_t core_vertices = 0, tree_vertices = 0, removed_parents = 0;
+  int64_t hd_root = -1;                   // highest-degree core vertex
 
   g500::ScanWorkspace scan_ws;
   g500::RadixWorkspace radix_ws;
@@ -58,10 +63,21 @@ void generate_edges(g500_graph* g, int scale, int edgefactor, uint64_t seed1,
                     uint64_t seed2);
 
 // construct.hip
+// CSR of `edges` into g (force_log_n < 0: estimate the vertex space).
+void build_csr(g500_graph* g, const uint2* edges, int64_t ne, bool symmetric,
+               int force_log_n);
 void construct_graph(g500_graph* g, int corebfs);
 void ensure_host_orig2new(g500_graph* g);
 
 // corebfs.hip
+constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 void corebfs_prune(g500_graph* g);
+// reachability reset of the forest after a BFS from hd_root over the core;
+// `visited` is that BFS's visited bitmap over core new IDs
+void corebfs_reset_unreachable(g500_graph* g, const uint64_t* visited,
+                               hipStream_t s);
+inline bool has_parent_host(const g500_graph* g, int64_t v) {
+  return g->corebfs && (g->h_has_parent[(size_t)(v >> 6)] >> (v & 63)) & 1ull;
+}
 
 }  // namespace g500

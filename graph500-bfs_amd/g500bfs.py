@@ -31,7 +31,8 @@ class GraphInfo(C.Structure):
                 ("n_c", C.c_int64), ("m", C.c_int64), ("m_sym", C.c_int64),
                 ("nedges_input", C.c_int64), ("max_used_vertex", C.c_int64),
                 ("max_degree", C.c_int64), ("core_vertices", C.c_int64),
-                ("tree_vertices", C.c_int64), ("device_bytes", C.c_int64)]
+                ("tree_vertices", C.c_int64), ("device_bytes", C.c_int64),
+                ("removed_parents", C.c_int64), ("hd_root", C.c_int64)]
 
 
 class LevelTrace(C.Structure):
@@ -65,6 +66,7 @@ def load():
         "g500_graph_has_edge": [vp, i64, C.POINTER(C.c_int)],
         "g500_graph_copy_edges": [vp, i64, i64, i64p, i64p],
         "g500_graph_copy_csr": [vp, i64p, u32p, i64p, u32p, u32p],
+        "g500_graph_copy_forest": [vp, i64p, C.POINTER(C.c_uint8)],
         "g500_find_roots": [vp, C.c_int, C.c_int, C.c_int, i64p, C.POINTER(C.c_int)],
         "g500_graph_free": [vp],
         "g500_make_random_numbers": [i64, C.c_uint64, C.c_uint64, i64, C.POINTER(C.c_double)],
@@ -163,6 +165,17 @@ class Graph:
                                           _p(deg, C.c_int64), _p(n2o, C.c_uint32),
                                           _p(o2n, C.c_uint32)))
         return dict(row_off=row, col=col[:inf.m], deg_new=deg, new2orig=n2o, orig2new=o2n)
+
+    def forest(self):
+        n = self.info.n
+        tp = np.empty(n, np.int64)
+        hp = np.empty(n, np.uint8)
+        _check(load().g500_graph_copy_forest(self.h, _p(tp, C.c_int64), _p(hp, C.c_uint8)))
+        return tp, hp
+
+    def refresh(self):
+        self._info = None
+        return self
 
     def find_roots(self, nroots, r1=0, r2=0):
         out = np.empty(max(nroots, 1), np.int64)

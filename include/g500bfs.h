@@ -81,6 +81,9 @@ typedef struct g500_graph_info {
   int64_t core_vertices;   /* CoreBFS: vertices kept in the 2-core, else 0 */
   int64_t tree_vertices;   /* CoreBFS: pruned vertices holding a parent */
   int64_t device_bytes;    /* device memory held by the graph */
+  int64_t removed_parents; /* CoreBFS: parents reset as unreachable (after the
+                              first g500_bfs_create), else 0 */
+  int64_t hd_root;         /* CoreBFS: highest-degree core vertex, else -1 */
 } g500_graph_info;
 
 /* ---- graph (L3/L4: generation + construction) -------------------------- */
@@ -128,6 +131,12 @@ int g500_graph_copy_edges(const g500_graph* g, int64_t start, int64_t count,
 int g500_graph_copy_csr(const g500_graph* g, int64_t* row_off, uint32_t* col,
                         int64_t* deg_new, uint32_t* new2orig,
                         uint32_t* orig2new);
+
+/* CoreBFS forest by original ID: tree_parent (n, -1 = none or reset) and
+ * has_parent (n bytes, 1 = a parent was assigned while pruning; has_edge
+ * counts these vertices).  Either pointer may be NULL. */
+int g500_graph_copy_forest(const g500_graph* g, int64_t* tree_parent,
+                           uint8_t* has_parent);
 
 /* Roots drawn as find_roots does (mpi/benchmark_helper.hpp:527-562) with
  * seeds (seed1 + r1, seed2 + r2); returns the count in *found. */

@@ -720,3 +720,187 @@ double orc_harmonic_mean_teps(const double* t, const double* te, int n) {
   for (int i = 0; i < n; ++i) s += t[i] / te[i];
   return 1.0 / (s / n);
 }
+
+/* ------------------------------------------------------------------------ */
+/* CoreBFS                                                                   */
+/* ------------------------------------------------------------------------ */
+int orc_corebfs_prune(const orc_graph* g, int64_t* tree_parent,
+                      uint8_t* has_parent, uint8_t* in_core, int64_t stats[4]) {
+  const int64_t n = g->n, nc = g->n_c;
+  int64_t* deg = (int64_t*)malloc((size_t)(nc ? nc : 1) * sizeof(int64_t));
+  uint8_t* alive = (uint8_t*)malloc((size_t)(nc ? nc : 1));
+  int64_t* par_new = (int64_t*)malloc((size_t)(nc ? nc : 1) * sizeof(int64_t));
+  int64_t* leaves = (int64_t*)malloc((size_t)(nc ? nc : 1) * sizeof(int64_t));
+  int64_t* twig_t = (int64_t*)malloc((size_t)(nc ? nc : 1) * sizeof(int64_t));
+  for (int64_t v = 0; v < nc; ++v) {
+    deg[v] = g->row_off[v + 1] - g->row_off[v];
+    alive[v] = deg[v] > 0;
+    par_new[v] = -1;
+  }
+  int iters = 0;
+  for (;;) {
+    int64_t nl = 0;
+    for (int64_t v = 0; v < nc; ++v)
+      if (deg[v] == 1) leaves[nl++] = v;
+    ++iters;
+    if (nl == 0) break;
+    /* twigs: first neighbour live at the iteration start */
+    for (int64_t i = 0; i < nl; ++i) {
+      const int64_t s = leaves[i];
+      twig_t[i] = -1;
+      for (int64_t e = g->row_off[s]; e < g->row_off[s + 1]; ++e)
+        if (alive[g->col[e]]) {
+          twig_t[i] = g->col[e];
+          break;
+        }
+    }
+    for (int64_t i = 0; i < nl; ++i) { /* update_by_source */
+      if (twig_t[i] < 0) continue;
+      const int64_t s = leaves[i];
+      deg[s] = 0;
+      par_new[s] = twig_t[i];
+      alive[s] = 0;
+    }
+    for (int64_t i = 0; i < nl; ++i) { /* update_by_target */
+      const int64_t t = twig_t[i];
+      if (t < 0 || deg[t] == 0) continue;
+      if (--deg[t] == 0) alive[t] = 0;
+    }
+  }
+  int64_t ncore = 0, mcore = 0, ntree = 0;
+  for (int64_t o = 0; o < n; ++o) {
+    tree_parent[o] = -1;
+    has_parent[o] = 0;
+    in_core[o] = 0;
+  }
+  for (int64_t v = 0; v < nc; ++v) {
+    const uint32_t o = g->new2orig[v];
+    if (par_new[v] >= 0) {
+      tree_parent[o] = g->new2orig[par_new[v]];
+      has_parent[o] = 1;
+      ++ntree;
+    }
+    if (alive[v]) {
+      in_core[o] = 1;
+      ++ncore;
+      for (int64_t e = g->row_off[v]; e < g->row_off[v + 1]; ++e)
+        if (alive[g->col[e]]) ++mcore;
+    }
+  }
+  stats[0] = ncore;
+  stats[1] = mcore;
+  stats[2] = ntree;
+  stats[3] = iters;
+  free(deg);
+  free(alive);
+  free(par_new);
+  free(leaves);
+  free(twig_t);
+  return 0;
+}
+
+int64_t orc_corebfs_reset_unreachable(const orc_graph* g, int64_t* tree_parent,
+                                      const uint8_t* has_parent,
+                                      const uint8_t* in_core, int64_t* hd_root) {
+  const int64_t n = g->n, nc = g->n_c;
+  /* highest core degree, ties by lowest original ID */
+  int64_t best = -1, best_deg = -1;
+  for (int64_t o = 0; o < n; ++o) {
+    if (!in_core[o]) continue;
+    const uint32_t v = g->orig2new[o];
+    int64_t d = 0;
+    for (int64_t e = g->row_off[v]; e < g->row_off[v + 1]; ++e)
+      if (in_core[g->new2orig[g->col[e]]]) ++d;
+    if (d > best_deg) {
+      best_deg = d;
+      best = o;
+    }
+  }
+  *hd_root = best;
+  uint8_t* reach = (uint8_t*)calloc((size_t)(n > 0 ? n : 1), 1);
+  if (best >= 0) { /* BFS inside the core */
+    uint32_t* q = (uint32_t*)malloc((size_t)(nc ? nc : 1) * sizeof(uint32_t));
+    int64_t h = 0, t = 0;
+    q[t++] = g->orig2new[best];
+    reach[best] = 1;
+    while (h < t) {
+      const uint32_t u = q[h++];
+      for (int64_t e = g->row_off[u]; e < g->row_off[u + 1]; ++e) {
+        const uint32_t w = g->col[e];
+        const uint32_t ow = g->new2orig[w];
+        if (!in_core[ow] || reach[ow]) continue;
+        reach[ow] = 1;
+        q[t++] = w;
+      }
+    }
+    free(q);
+  }
+  /* reachability flows from parents to children through the forest */
+  for (int changed = 1; changed;) {
+    changed = 0;
+    for (int64_t o = 0; o < n; ++o)
+      if (has_parent[o] && !reach[o] && tree_parent[o] >= 0 && reach[tree_parent[o]]) {
+        reach[o] = 1;
+        changed = 1;
+      }
+  }
+  int64_t removed = 0;
+  for (int64_t o = 0; o < n; ++o)
+    if (has_parent[o] && !reach[o]) {
+      tree_parent[o] = -1;
+      ++removed;
+    }
+  free(reach);
+  return removed;
+}
+
+int orc_corebfs_run(const orc_graph* g, const int64_t* tree_parent,
+                    const uint8_t* has_parent, const uint8_t* in_core,
+                    int64_t root, int64_t* pred) {
+  const int64_t n = g->n, nc = g->n_c;
+  for (int64_t o = 0; o < n; ++o) pred[o] = -1;
+  /* walk to the core: (vertex, entry) pairs along the path */
+  int64_t pv[256], pe[256];
+  int np = 0;
+  int64_t u = root, p = root;
+  int lv = 0;
+  for (;; ++lv) {
+    if (np < 256) {
+      pv[np] = p;
+      pe[np] = PRED_ENTRY(u, lv);
+      ++np;
+    }
+    u = p;
+    p = has_parent[u] ? tree_parent[u] : -1;
+    if (p < 0) break;
+  }
+  const int64_t core_root = u;
+  /* BFS inside the core from core_root at level lv (core vertices only) */
+  if (in_core[core_root]) {
+    uint32_t* q = (uint32_t*)malloc((size_t)(nc ? nc : 1) * sizeof(uint32_t));
+    int32_t* lev = (int32_t*)malloc((size_t)(nc ? nc : 1) * sizeof(int32_t));
+    for (int64_t v = 0; v < nc; ++v) lev[v] = -1;
+    int64_t h = 0, t = 0;
+    const uint32_t r = g->orig2new[core_root];
+    q[t++] = r;
+    lev[r] = lv;
+    pred[core_root] = core_root;
+    while (h < t) {
+      const uint32_t x = q[h++];
+      for (int64_t e = g->row_off[x]; e < g->row_off[x + 1]; ++e) {
+        const uint32_t w = g->col[e];
+        if (!in_core[g->new2orig[w]] || lev[w] >= 0) continue;
+        lev[w] = lev[x] + 1;
+        pred[g->new2orig[w]] = PRED_ENTRY(g->new2orig[x], lev[w]);
+        q[t++] = w;
+      }
+    }
+    free(q);
+    free(lev);
+  }
+  /* tree parents with unknown depth, then the walked path */
+  for (int64_t o = 0; o < n; ++o)
+    if (has_parent[o]) pred[o] = tree_parent[o] < 0 ? -1 : PRED_ENTRY(tree_parent[o], 0xFFFF);
+  for (int i = 0; i < np; ++i) pred[pv[i]] = pe[i];
+  return lv;
+}

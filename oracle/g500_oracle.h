@@ -142,6 +142,35 @@ int orc_validate(const int64_t* v0, const int64_t* v1, int64_t nedges,
                  int64_t root, int64_t* pred, int64_t* te_out,
                  int64_t errs[ORC_ERR_NUM]);
 
+/* ---- CoreBFS (mpi/corebfs_adaptor.hpp, external/indexed_bfs/bfs/corebfs) */
+/* Iterative leaf peeling of prune_trees (decomposition.hpp:350-572) on the
+ * deduplicated symmetric graph.  Per iteration: leaves = live degree 1; each
+ * leaf takes its one neighbour that is live at the iteration start as parent
+ * (find_mark_twigs); then leaves drop to degree 0 (update_by_source) and each
+ * parent loses one degree, dying at 0 (update_by_target).
+ * Outputs by original ID: tree_parent (-1 none), has_parent (a parent was
+ * assigned), in_core (live degree > 0 at the end).  stats = {core vertices,
+ * core directed edges, vertices with a parent, iterations}. */
+int orc_corebfs_prune(const orc_graph* g, int64_t* tree_parent,
+                      uint8_t* has_parent, uint8_t* in_core, int64_t stats[4]);
+/* corebfs_index::reset_unreachable (corebfs_adaptor.hpp:228-249 +
+ * postprocess.hpp:106-154): BFS over the core from its highest-degree vertex
+ * (ties: lowest original ID), then tree parents whose vertex cannot reach
+ * that component through the forest become -1 (has_parent stays set).
+ * Returns the number of parents removed; *hd_root = the BFS root. */
+int64_t orc_corebfs_reset_unreachable(const orc_graph* g, int64_t* tree_parent,
+                                      const uint8_t* has_parent,
+                                      const uint8_t* in_core, int64_t* hd_root);
+/* BfsBase::run_bfs with CoreBFS (bfs.hpp:3214-3232): walk up the forest from
+ * root (bfs_tree_with_callback, corebfs.hpp:125-157), BFS inside the core from
+ * the first core vertex at the walked distance, then write every tree
+ * vertex's parent with depth 0xFFFF and the walked path with its depths
+ * (write_tree_parents, corebfs_adaptor.hpp:285-301).  pred as the
+ * reference leaves it before validation.  Returns the core root's level. */
+int orc_corebfs_run(const orc_graph* g, const int64_t* tree_parent,
+                    const uint8_t* has_parent, const uint8_t* in_core,
+                    int64_t root, int64_t* pred);
+
 /* ---- roots / metric ---------------------------------------------------- */
 /* benchmark_helper.hpp:527-562 (single rank).  Returns the root count. */
 int orc_find_roots(const orc_graph* g, int nroots, int r1, int r2,

@@ -80,6 +80,11 @@ def lib():
                                       C.POINTER(C.c_int8), C.c_int]
         L.orc_validate.argtypes = [i64p, i64p, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                    C.c_int64, i64p, i64p, i64p]
+        u8p = C.POINTER(C.c_uint8)
+        L.orc_corebfs_prune.argtypes = [C.c_void_p, i64p, u8p, u8p, i64p]
+        L.orc_corebfs_reset_unreachable.argtypes = [C.c_void_p, i64p, u8p, u8p, i64p]
+        L.orc_corebfs_reset_unreachable.restype = C.c_int64
+        L.orc_corebfs_run.argtypes = [C.c_void_p, i64p, u8p, u8p, C.c_int64, i64p]
         L.orc_find_roots.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, i64p]
         L.orc_harmonic_mean_teps.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
         L.orc_harmonic_mean_teps.restype = C.c_double
@@ -176,6 +181,29 @@ class Graph:
         out = np.empty(nroots, np.int64)
         k = lib().orc_find_roots(self.h, nroots, r1, r2, _p(out, C.c_int64))
         return out[:k].copy()
+
+    def corebfs(self):
+        """Prune + reset_unreachable; returns a dict of arrays and stats."""
+        n = self.info.n
+        tp = np.empty(n, np.int64)
+        hp = np.empty(n, np.uint8)
+        ic = np.empty(n, np.uint8)
+        st = np.empty(4, np.int64)
+        lib().orc_corebfs_prune(self.h, _p(tp, C.c_int64), _p(hp, C.c_uint8), _p(ic, C.c_uint8),
+                                _p(st, C.c_int64))
+        hd = np.zeros(1, np.int64)
+        removed = lib().orc_corebfs_reset_unreachable(self.h, _p(tp, C.c_int64), _p(hp, C.c_uint8),
+                                                      _p(ic, C.c_uint8), _p(hd, C.c_int64))
+        return dict(tree_parent=tp, has_parent=hp, in_core=ic, core_vertices=int(st[0]),
+                    core_edges=int(st[1]), tree_vertices=int(st[2]), iterations=int(st[3]),
+                    removed=int(removed), hd_root=int(hd[0]))
+
+    def corebfs_run(self, cb, root):
+        pred = np.empty(self.info.n, np.int64)
+        lv = lib().orc_corebfs_run(self.h, _p(cb["tree_parent"], C.c_int64),
+                                   _p(cb["has_parent"], C.c_uint8), _p(cb["in_core"], C.c_uint8),
+                                   int(root), _p(pred, C.c_int64))
+        return pred, lv
 
     def validate(self, root, pred, corebfs=False, nglobal=None):
         pred = np.ascontiguousarray(pred, np.int64).copy()

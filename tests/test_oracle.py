@@ -333,3 +333,62 @@ def test_harmonic_mean():
     hm = O.lib().orc_harmonic_mean_teps(t.ctypes.data_as(C.POINTER(C.c_double)),
                                         e.ctypes.data_as(C.POINTER(C.c_double)), 3)
     assert abs(hm - 3.0 / (0.1 + 0.2 + 0.4)) < 1e-12
+
+
+# ---------------------------------------------------------------- CoreBFS
+def test_corebfs_matches_reference_log_scale20(golden):
+    v0, v1 = O.generate_edges(20)
+    g = O.Graph(v0, v1)
+    cb = g.corebfs()
+    ref = golden["corebfs_scale20"]
+    assert cb["core_vertices"] == ref["core_vertices"]
+    assert cb["core_edges"] == ref["core_edges"]
+    assert cb["removed"] == ref["removed_parents"]
+    assert cb["hd_root"] == ref["highest_degree_vertex"]
+
+
+def _core_has_edge(g, cb, v):
+    return bool(cb["in_core"][v]) or bool(cb["has_parent"][v])
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_corebfs_levels_equal_plain_bfs(scale, golden):
+    v0, v1 = O.generate_edges(scale)
+    g = O.Graph(v0, v1)
+    cb = g.corebfs()
+    roots = [r for r in g.find_roots(8)]
+    for r in roots:
+        pred, lv = g.corebfs_run(cb, int(r))
+        # tree vertices carry depth 0xFFFF until the validator rebuilds them
+        ok, te, errs = O.validate(g.v0, g.v1, g.info.max_used_vertex + 1, g.info.n, int(r),
+                                  pred, True)
+        assert ok, errs
+        assert te == golden["pf_nedge"][scale]
+        d = (pred.astype(np.uint64) >> np.uint64(48)).astype(np.int64)
+        d[d == 0xFFFF] = -1
+        assert np.array_equal(d, g.bfs(int(r)))
+
+
+def test_corebfs_small_forest_by_hand():
+    # cycle 0-1-2-0 with a path 2-3-4 and a leaf 1-5; separate edge 7-8;
+    # separate star 10-{11,12,13}
+    e = [(0, 1), (1, 2), (2, 0), (2, 3), (3, 4), (1, 5), (7, 8), (10, 11), (10, 12), (10, 13)]
+    v0 = np.array([a for a, b in e], np.int64)
+    v1 = np.array([b for a, b in e], np.int64)
+    g = O.Graph(v0, v1)
+    cb = g.corebfs()
+    tp, hp, ic = cb["tree_parent"], cb["has_parent"], cb["in_core"]
+    assert ic[[0, 1, 2]].all() and not ic[[3, 4, 5, 7, 8, 10, 11]].any()
+    assert tp[4] == 3 and tp[3] == 2 and tp[5] == 1          # forest toward the core
+    assert hp[7] and hp[8] and tp[7] == -1 and tp[8] == -1   # two-vertex component reset
+    assert hp[11] and tp[11] == -1 and not hp[10]            # star: leaves reset, centre none
+    assert cb["removed"] == 5 and cb["iterations"] == 3
+    pred, lv = g.corebfs_run(cb, 4)                          # root deep in a tree
+    assert lv == 2                                           # 4 -> 3 -> 2 (core)
+    assert (pred[4] & 0xFFFFFFFFFFFF) == 4 and (pred[4] >> 48) == 0
+    assert (pred[3] & 0xFFFFFFFFFFFF) == 4 and (pred[3] >> 48) == 1
+    assert (pred[2] & 0xFFFFFFFFFFFF) == 3 and (pred[2] >> 48) == 2
+    assert (pred[5] >> 48) & 0xFFFF == 0xFFFF and (pred[5] & 0xFFFFFFFFFFFF) == 1
+    ok, te, errs = O.validate(v0, v1, 14, g.info.n, 4, pred, True)
+    assert ok, errs
+    assert (pred[5] >> 48) == 4 and (pred[0] >> 48) == 3     # rebuilt depths
