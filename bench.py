@@ -50,6 +50,21 @@ def log(msg):
 PF_NEDGE = None
 
 
+def profiled_traffic(args):
+    """HBM bytes per BFS from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this same workload (None for any other workload)."""
+    if (args.scale, args.edgefactor, args.alpha, args.beta) != (26, 16, 10.0, 14.0):
+        return None
+    path = os.path.join(ROOT, "profiles", "bfs_scale26_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_bfs:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def pf_nedge(scale):
     global PF_NEDGE
     if PF_NEDGE is None:
@@ -226,7 +241,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(args),
+            "traffic_unit": "bytes per BFS (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "kernel": "one root's BFS (all level kernels on the engine stream, HIP events)",
             "bytes_per_te": round(b_per_te, 3),
             "mean_bfs_gpu_ms": round(mean_gpu_s * 1e3, 4),

@@ -357,6 +357,14 @@ int g500_bfs_set_timing(g500_bfs* b, int on) {
   });
 }
 
+int g500_bfs_set_exec_alpha(g500_bfs* b, double exec_alpha) {
+  return guard([&] {
+    if (!b) g500::fail(G500_ERR_ARG, "null bfs handle");
+    if (!(exec_alpha >= 0)) g500::fail(G500_ERR_ARG, "exec_alpha must be >= 0");
+    b->exec_alpha = exec_alpha;
+  });
+}
+
 int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st, g500_level_trace* levels,
                        int max_levels) {
   return guard([&] {

@@ -33,6 +33,7 @@
 // stream synchronisations per level.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -192,43 +193,71 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     if (use_win)
       for (int64_t j = t; j < wn; j += kT) win[j] = dsum[lo + j];
     __syncthreads();
-    for (int it = 0; it < kTdItems; ++it) {
-      const uint64_t e = e0 + (uint64_t)it * kT + t;
-      bool won = false;
-      uint32_t v = 0;
-      if (e < e1) {
-        int64_t i;
-        uint64_t base;
-        if (use_win) {
-          const int64_t li = owner_search(win, 0, wn - 1, e);
-          i = lo + li;
-          base = win[li];
-        } else {
-          i = owner_search(dsum, lo, hi, e);
-          base = dsum[i];
+    // item j of lane t is edge e0 + j*kT + t (coalesced col reads); the loads
+    // of all items are issued phase by phase so their latencies overlap
+    uint32_t v[kTdItems], pu[kTdItems];
+    bool act[kTdItems];
+    {
+      int64_t kk[kTdItems];
+#pragma unroll
+      for (int j = 0; j < kTdItems; ++j) {
+        const uint64_t e = e0 + (uint64_t)j * kT + t;
+        act[j] = e < e1;
+        int64_t i = lo;
+        uint64_t base = 0;
+        if (act[j]) {
+          if (use_win) {
+            const int64_t li = owner_search(win, 0, wn - 1, e);
+            i = lo + li;
+            base = win[li];
+          } else {
+            i = owner_search(dsum, lo, hi, e);
+            base = dsum[i];
+          }
         }
-        const uint32_t u = q[i];
-        v = col[row_off[u] + (int64_t)(e - base)];
-        const unsigned long long bit = 1ull << (v & 63);
-        unsigned long long* wp = vis + (v >> 6);
-        if (!(__atomic_load_n(wp, __ATOMIC_RELAXED) & bit)) {
-          const unsigned long long old = atomicOr(wp, bit);
-          won = !(old & bit);
-        }
-        if (won) {
-          pred_new[v] = (int64_t)((uint64_t)n2o[u] | level_bits);
-          my_edges += deg_new[v];
-        }
+        const uint32_t u = act[j] ? q[i] : 0u;
+        kk[j] = act[j] ? row_off[u] + (int64_t)(e - base) : 0;
+        pu[j] = act[j] ? n2o[u] : 0u;
       }
-      const uint64_t mask = __ballot(won);
-      if (mask) {
-        uint32_t wbase = 0;
-        if ((mask & lanemask_lt()) == 0 && won)
-          wbase = atomicAdd(&lq_n, (uint32_t)__popcll(mask));
-        const int leader = __ffsll((long long)mask) - 1;
-        wbase = __shfl(wbase, leader, kWave);
-        if (won) lq[wbase + __popcll(mask & lanemask_lt())] = v;
+#pragma unroll
+      for (int j = 0; j < kTdItems; ++j) v[j] = act[j] ? col[kk[j]] : 0u;
+    }
+    unsigned long long w[kTdItems];
+#pragma unroll
+    for (int j = 0; j < kTdItems; ++j)
+      w[j] = act[j] ? __atomic_load_n(vis + (v[j] >> 6), __ATOMIC_RELAXED) : ~0ull;
+    uint32_t nwon = 0;
+    bool won[kTdItems];
+#pragma unroll
+    for (int j = 0; j < kTdItems; ++j) {
+      const unsigned long long bit = 1ull << (v[j] & 63);
+      won[j] = false;
+      if (act[j] && !(w[j] & bit)) won[j] = !(atomicOr(vis + (v[j] >> 6), bit) & bit);
+      nwon += won[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kTdItems; ++j)
+      if (won[j]) {
+        pred_new[v[j]] = (int64_t)((uint64_t)pu[j] | level_bits);
+        my_edges += deg_new[v[j]];
       }
+    // append winners to the block's LDS queue: wave prefix + one LDS atomic
+    {
+      const int lane = lane_id();
+      uint32_t incl = nwon;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += y;
+      }
+      const uint32_t wtot = __shfl(incl, kWave - 1, kWave);
+      uint32_t wbase = 0;
+      if (lane == kWave - 1 && wtot) wbase = atomicAdd(&lq_n, wtot);
+      wbase = __shfl(wbase, kWave - 1, kWave);
+      uint32_t pos = wbase + incl - nwon;
+#pragma unroll
+      for (int j = 0; j < kTdItems; ++j)
+        if (won[j]) lq[pos++] = v[j];
     }
     __syncthreads();
     const uint32_t cnt = lq_n;
@@ -457,6 +486,7 @@ void bfs_init(g500_bfs* b) {
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
   ensure_host_orig2new(g);
+  if (const char* ea = getenv("G500_EXEC_ALPHA")) b->exec_alpha = atof(ea);
   if (g->corebfs && !g->core_reset_done) {
     // corebfs_index::reset_unreachable: one BFS over the core from its
     // highest-degree vertex decides which forest parents survive
@@ -576,8 +606,26 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       G500_CHECK_HIP(hipEventRecord(lev0, s));
     }
     const uint64_t level_bits = (uint64_t)level << 48;
-    const unsigned long long seq = ++b->seq;
-    if (td) {
+    unsigned long long seq = ++b->seq;
+    // a top-down level with a very large frontier runs as a bottom-up sweep
+    // followed by the bitmap -> queue conversion (same vertices and depths)
+    const bool exec_bu =
+        !td || (b->exec_alpha > 0 && est_edges > 0 &&
+                (double)est_edges > (double)g->m / b->exec_alpha);
+    if (td && exec_bu) {
+      hipLaunchKernelGGL(k_bu_step, dim3(grid_for(ceil_div(words, kWave), kWavesPerBlock)),
+                         dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(), words,
+                         g->row_off.get(), g->col.get(), g->first.get(),
+                         g->new2orig.get(), b->pred_new.get(), ls, b->d_report, seq,
+                         level_bits);
+      G500_CHECK_LAUNCH();
+      std::swap(b->vis.p, b->vis_alt.p);
+      seq = ++b->seq;
+      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)ceil_div(words, kT)), dim3(kT),
+                         0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
+                         b->nq.get(), ls, b->d_report, seq);
+      G500_CHECK_LAUNCH();
+    } else if (td) {
       if (qlen > 0) {
         hipLaunchKernelGGL(k_td_degrees, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT),
                            0, s, b->q.get(), qlen, g->row_off.get(), b->dsum.get());
@@ -609,6 +657,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     tr.top_down = td ? 1 : 0;
     tr.nq_size = nq_size;
     tr.nq_edges = nq_edges;
+    tr.exec_bottom_up = exec_bu ? 1 : 0;
     b->trace.push_back(tr);
     if (nq_size == 0) break;
 

@@ -42,6 +42,7 @@ struct g500_bfs {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
   bool timing = false;
+  double exec_alpha = 0.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
 
   bool have_result = false;
   int64_t last_root = -1;

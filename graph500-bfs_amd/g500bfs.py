@@ -37,7 +37,7 @@ class GraphInfo(C.Structure):
 
 class LevelTrace(C.Structure):
     _fields_ = [("level", C.c_int), ("top_down", C.c_int), ("nq_size", C.c_int64),
-                ("nq_edges", C.c_int64), ("gpu_ms", C.c_float)]
+                ("nq_edges", C.c_int64), ("gpu_ms", C.c_float), ("exec_bottom_up", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -76,6 +76,7 @@ def load():
         "g500_bfs_get_levels": [vp, C.POINTER(C.c_uint16)],
         "g500_bfs_validate": [vp, i64, i64p, i64p, i64p, C.POINTER(C.c_int)],
         "g500_bfs_set_timing": [vp, C.c_int],
+        "g500_bfs_set_exec_alpha": [vp, C.c_double],
         "g500_bfs_get_stats": [vp, C.POINTER(RunStats), C.POINTER(LevelTrace), C.c_int],
         "g500_bfs_destroy": [vp],
     }
@@ -233,6 +234,9 @@ class Bfs:
                                         _p(errs, C.c_int64), C.byref(ok)))
         return bool(ok.value), te.value, dict(zip(VERR_NAMES, errs.tolist()))
 
+    def set_exec_alpha(self, exec_alpha):
+        _check(load().g500_bfs_set_exec_alpha(self.h, float(exec_alpha)))
+
     def set_timing(self, on=True):
         _check(load().g500_bfs_set_timing(self.h, int(on)))
 
@@ -241,7 +245,8 @@ class Bfs:
         tr = (LevelTrace * max_levels)()
         _check(load().g500_bfs_get_stats(self.h, C.byref(st), tr, max_levels))
         levels = [dict(level=tr[i].level, top_down=tr[i].top_down, nq_size=tr[i].nq_size,
-                       nq_edges=tr[i].nq_edges, gpu_ms=tr[i].gpu_ms)
+                       nq_edges=tr[i].nq_edges, gpu_ms=tr[i].gpu_ms,
+                       exec_bottom_up=tr[i].exec_bottom_up)
                   for i in range(min(st.nlevels, max_levels))]
         return st, levels
 

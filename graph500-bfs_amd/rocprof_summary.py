@@ -1,0 +1,95 @@
+"""Per-BFS summary of rocprofv3 CSV output for the plain BFS engine.
+
+A BFS run is the span of kernels from one k_bfs_init to the k_pred_gather /
+k_write_path that ends it.  The kernel-trace pass gives time per kernel per
+run; the --pmc passes give FETCH_SIZE and WRITE_SIZE per kernel per run, each
+scaled by what the same counter reads for a device-to-device copy of known
+size (pass the calibration runs' CSV files).
+
+usage:
+  python rocprof_summary.py times  KT_KERNEL_TRACE.csv
+  python rocprof_summary.py traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_BYTES
+"""
+import collections
+import csv
+import statistics
+import sys
+
+
+def short(name):
+    return name.replace("(anonymous namespace)::", "").split("(")[0].split("::")[-1]
+
+
+def _end_of_run(names):
+    return max(i for i, n in enumerate(names) if "k_pred_gather" in n or "k_write_path" in n)
+
+
+def runs(rows, name_of):
+    """Split dispatch-ordered rows into BFS runs (drops the trailing segment)."""
+    idx = [i for i, r in enumerate(rows) if "k_bfs_init" in name_of(r)]
+    out = []
+    for a, b in zip(idx, idx[1:]):
+        seg = rows[a:b]
+        out.append(seg[: _end_of_run([name_of(r) for r in seg]) + 1])
+    return out
+
+
+def times(path):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    spans, per = [], collections.defaultdict(list)
+    for seg in runs(rows, lambda r: r["Kernel_Name"]):
+        spans.append((int(seg[-1]["End_Timestamp"]) - int(seg[0]["Start_Timestamp"])) / 1e3)
+        c = collections.Counter()
+        for r in seg:
+            c[short(r["Kernel_Name"])] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        for k in set(per) | set(c):
+            per[k].append(c.get(k, 0.0))
+    print(f"BFS runs: {len(spans)}; first-kernel-start to last-kernel-end per run: "
+          f"mean {statistics.mean(spans):.1f} us, median {statistics.median(spans):.1f} us")
+    print(f"kernel-busy time per run: mean {sum(statistics.mean(v) for v in per.values()):.1f} us")
+    for k, v in sorted(per.items(), key=lambda kv: -statistics.mean(kv[1])):
+        print(f"  {k:26s} {statistics.mean(v):9.1f} us/run")
+
+
+def _counter(path, name):
+    out = []
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == name:
+            out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    return sorted(out)
+
+
+def traffic(fetch, write, cfetch, cwrite, copy_bytes):
+    kib = copy_bytes / 1024.0
+    cf = statistics.mean(c[2] for c in _counter(cfetch, "FETCH_SIZE") if "copyBuffer" in c[1])
+    cw = statistics.mean(c[2] for c in _counter(cwrite, "WRITE_SIZE") if "copyBuffer" in c[1])
+    ff, fw = kib / cf, kib / cw
+    print(f"calibration: a {copy_bytes} B copy reads FETCH_SIZE {cf:.1f} KiB (x{ff:.3f}), "
+          f"WRITE_SIZE {cw:.1f} KiB (x{fw:.3f})")
+    res = {}
+    for label, path, cname, f in (("fetch", fetch, "FETCH_SIZE", ff), ("write", write, "WRITE_SIZE", fw)):
+        tot, byk = [], collections.defaultdict(list)
+        for seg in runs(_counter(path, cname), lambda c: c[1]):
+            t = collections.Counter()
+            for c in seg:
+                t[short(c[1])] += c[2] * 1024.0 * f
+            tot.append(sum(t.values()))
+            for k in set(byk) | set(t):
+                byk[k].append(t.get(k, 0.0))
+        res[label] = (tot, byk)
+    ft, fk = res["fetch"]
+    wt, wk = res["write"]
+    f_med, w_med = statistics.median(ft), statistics.median(wt)
+    print(f"per run HBM bytes (median over {len(ft)} runs): fetch {f_med/1e9:.3f} GB, write {w_med/1e9:.3f} GB")
+    print(f"traffic_bytes_per_bfs: {int(f_med + w_med)}")
+    keys = sorted(set(fk) | set(wk), key=lambda k: -(statistics.median(fk.get(k, [0])) + statistics.median(wk.get(k, [0]))))
+    for k in keys:
+        print(f"  {k:26s} fetch {statistics.median(fk.get(k, [0]))/1e6:9.1f} MB  write {statistics.median(wk.get(k, [0]))/1e6:9.1f} MB")
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "times":
+        times(sys.argv[2])
+    else:
+        traffic(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], int(sys.argv[6]))

@@ -182,10 +182,11 @@ int g500_bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
 /* Per-level trace of the last run. */
 typedef struct g500_level_trace {
   int level;        /* BFS level number (root = level 0 of the core BFS) */
-  int top_down;     /* 1 top-down, 0 bottom-up */
+  int top_down;     /* direction the reference's switch chose: 1 top-down */
   int64_t nq_size;  /* vertices discovered at this level */
   int64_t nq_edges; /* sum of their degrees (top-down levels only) */
   float gpu_ms;     /* device time of the level, 0 unless timing is on */
+  int exec_bottom_up; /* 1 if the level ran as a bottom-up sweep */
 } g500_level_trace;
 
 typedef struct g500_run_stats {
@@ -199,6 +200,14 @@ typedef struct g500_run_stats {
 /* Enables per-level HIP-event timing (off by default; costs one event pair
  * per level). */
 int g500_bfs_set_timing(g500_bfs* b, int on);
+
+/* Execution choice for top-down levels.  The direction decision and
+ * auto_tuning_data always follow the reference's alpha/beta rule; a level
+ * the rule runs top-down is executed as a bottom-up sweep when its frontier
+ * holds more than m / exec_alpha edges (same vertices, same depths, any valid
+ * parents).  0 disables (every top-down level runs top-down).  The default
+ * comes from G500_EXEC_ALPHA in the environment, else 0. */
+int g500_bfs_set_exec_alpha(g500_bfs* b, double exec_alpha);
 int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st,
                        g500_level_trace* levels, int max_levels);
 

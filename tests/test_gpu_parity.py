@@ -191,3 +191,29 @@ def test_validator_agrees_on_planted_mistakes(G):
         assert first(e_d) == first(e_o), (e_d, e_o)
     ok, te, _ = bfs.validate(r, host_pred=pred)
     assert ok and te == 65535
+
+
+@pytest.mark.parametrize("exec_alpha", [0.0, 1e9])
+def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
+    """Top-down levels run as top-down (0) or as bottom-up sweeps (1e9): the
+    levels, the direction schedule and auto_tuning_data stay identical."""
+    print(f"\n[exec] alpha_exec={exec_alpha}", flush=True)
+    g = G.Graph.generate(16).construct()
+    o0, o1 = O.generate_edges(16)
+    og = O.Graph(o0, o1)
+    bfs = G.Bfs(g)
+    bfs.set_exec_alpha(exec_alpha)
+    for r in g.find_roots(6):
+        r = int(r)
+        at = bfs.run(r)
+        assert np.array_equal(bfs.levels(), _oracle_levels(og, r))
+        ok, te, errs = bfs.validate(r)
+        assert ok and te == golden["pf_nedge"][16], errs
+        _, at_o, dirs_o = og.do_bfs(r)
+        st, trace = bfs.stats()
+        assert [t["top_down"] for t in trace] == dirs_o.tolist()
+        assert at.tolist() == at_o.tolist()
+        if exec_alpha == 0.0:
+            assert all(t["exec_bottom_up"] == (0 if t["top_down"] else 1) for t in trace)
+        else:
+            assert all(t["exec_bottom_up"] == 1 for t in trace[1:])
