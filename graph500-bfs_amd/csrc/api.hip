@@ -9,6 +9,7 @@
 
 #include "engine.hpp"
 #include "mrg.hpp"
+#include "msbfs.hpp"
 
 namespace g500 {
 
@@ -70,6 +71,15 @@ void need_graph(const g500_graph* g, bool constructed) {
     g500::fail(G500_ERR_STATE, "graph not constructed (call g500_graph_construct)");
 }
 
+void need_msbfs(g500_msbfs* b) {
+  if (!b) g500::fail(G500_ERR_ARG, "null batched-BFS handle");
+  G500_CHECK_HIP(hipSetDevice(b->g->device));
+}
+void need_result(g500_msbfs* b) {
+  need_msbfs(b);
+  if (b->in_flight) g500::fail(G500_ERR_STATE, "batch still in flight (call g500_msbfs_wait)");
+  if (!b->have_result) g500::fail(G500_ERR_STATE, "no completed batch");
+}
 bool has_edge_host(g500_graph* g, int64_t v) {
   if (v < 0 || v >= g->n) return false;
   g500::ensure_host_orig2new(g);
@@ -381,6 +391,122 @@ int g500_bfs_destroy(g500_bfs* b) {
     if (!b) return;
     (void)hipSetDevice(b->g->device);
     (void)hipStreamSynchronize(b->stream);
+    delete b;
+  });
+}
+
+int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out) {
+  return guard([&] {
+    need_graph(g, true);
+    if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (max_roots < 1 || max_roots > G500_MSBFS_MAX_ROOTS)
+      g500::fail(G500_ERR_ARG, "max_roots %d outside [1, %d]", max_roots, G500_MSBFS_MAX_ROOTS);
+    if (g->corebfs)
+      g500::fail(G500_ERR_STATE, "batched BFS needs a graph constructed without CoreBFS");
+    use_device(g->device);
+    auto* b = new g500_msbfs;
+    try {
+      b->g = g;
+      g500::msbfs_init(b, max_roots);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int g500_msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning* t) {
+  return guard([&] {
+    need_msbfs(b);
+    if (!t) g500::fail(G500_ERR_ARG, "null tuning");
+    if (b->in_flight) g500::fail(G500_ERR_STATE, "batch in flight");
+    g500::msbfs_set_tuning(b, *t);
+  });
+}
+
+int g500_msbfs_get_tuning(g500_msbfs* b, g500_msbfs_tuning* t) {
+  return guard([&] {
+    need_msbfs(b);
+    if (!t) g500::fail(G500_ERR_ARG, "null tuning");
+    *t = b->tuning;
+  });
+}
+
+int g500_msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots) {
+  return guard([&] {
+    need_msbfs(b);
+    g500::msbfs_launch(b, roots, nroots);
+  });
+}
+
+int g500_msbfs_query(g500_msbfs* b, int* done) {
+  return guard([&] {
+    need_msbfs(b);
+    if (!done) g500::fail(G500_ERR_ARG, "null output pointer");
+    *done = g500::msbfs_query(b) ? 1 : 0;
+  });
+}
+
+int g500_msbfs_wait(g500_msbfs* b) {
+  return guard([&] {
+    need_msbfs(b);
+    g500::msbfs_wait(b);
+  });
+}
+
+int g500_msbfs_run(g500_msbfs* b, const int64_t* roots, int nroots) {
+  return guard([&] {
+    need_msbfs(b);
+    g500::msbfs_launch(b, roots, nroots);
+    g500::msbfs_wait(b);
+  });
+}
+
+int g500_msbfs_get_levels(g500_msbfs* b, int index, uint8_t* host_levels) {
+  return guard([&] {
+    need_result(b);
+    if (!host_levels) g500::fail(G500_ERR_ARG, "null output buffer");
+    if (index < 0 || index >= b->nroots)
+      g500::fail(G500_ERR_ARG, "root index %d outside [0, %d)", index, b->nroots);
+    const size_t n = (size_t)b->g->n;
+    G500_CHECK_HIP(hipMemcpy(host_levels, b->out.get() + (size_t)index * n, n,
+                             hipMemcpyDeviceToHost));
+  });
+}
+
+int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride) {
+  return guard([&] {
+    need_result(b);
+    if (!dev_ptr || !row_stride) g500::fail(G500_ERR_ARG, "null output pointer");
+    *dev_ptr = b->out.get();
+    *row_stride = b->g->n;
+  });
+}
+
+int g500_msbfs_get_te(g500_msbfs* b, int64_t* te) {
+  return guard([&] {
+    need_result(b);
+    if (!te) g500::fail(G500_ERR_ARG, "null output buffer");
+    G500_CHECK_HIP(hipMemcpy(te, b->te.get(), (size_t)b->nroots * sizeof(int64_t),
+                             hipMemcpyDeviceToHost));
+  });
+}
+
+int g500_msbfs_get_stats(g500_msbfs* b, g500_msbfs_stats* st) {
+  return guard([&] {
+    need_result(b);
+    if (!st) g500::fail(G500_ERR_ARG, "null output pointer");
+    *st = b->stats;
+  });
+}
+
+int g500_msbfs_destroy(g500_msbfs* b) {
+  return guard([&] {
+    if (!b) return;
+    (void)hipSetDevice(b->g->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     delete b;
   });
 }

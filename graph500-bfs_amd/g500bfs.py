@@ -45,6 +45,20 @@ class RunStats(C.Structure):
                 ("bu_ms", C.c_double), ("td_edges", C.c_int64), ("root_level", C.c_int64)]
 
 
+class MsbfsTuning(C.Structure):
+    _fields_ = [("heavy_degree", C.c_int64), ("medium_degree", C.c_int64),
+                ("medium_group", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("use_graph", C.c_int32), ("heavy_chunk", C.c_int32),
+                ("td_alpha", C.c_double), ("stream_frac", C.c_double)]
+
+
+class MsbfsStats(C.Structure):
+    _fields_ = [("nroots", C.c_int32), ("nlevels", C.c_int32), ("gpu_ms", C.c_double),
+                ("discovered", C.c_int64), ("lane_discoveries", C.c_int64 * 64)]
+
+
+MSBFS_MAX_ROOTS = 256
+
 _lib = None
 
 
@@ -79,6 +93,18 @@ def load():
         "g500_bfs_set_exec_alpha": [vp, C.c_double],
         "g500_bfs_get_stats": [vp, C.POINTER(RunStats), C.POINTER(LevelTrace), C.c_int],
         "g500_bfs_destroy": [vp],
+        "g500_msbfs_create": [vp, C.c_int, C.POINTER(vp)],
+        "g500_msbfs_set_tuning": [vp, C.POINTER(MsbfsTuning)],
+        "g500_msbfs_get_tuning": [vp, C.POINTER(MsbfsTuning)],
+        "g500_msbfs_launch": [vp, i64p, C.c_int],
+        "g500_msbfs_query": [vp, C.POINTER(C.c_int)],
+        "g500_msbfs_wait": [vp],
+        "g500_msbfs_run": [vp, i64p, C.c_int],
+        "g500_msbfs_get_levels": [vp, C.c_int, C.POINTER(C.c_uint8)],
+        "g500_msbfs_device_levels": [vp, C.POINTER(vp), i64p],
+        "g500_msbfs_get_te": [vp, i64p],
+        "g500_msbfs_get_stats": [vp, C.POINTER(MsbfsStats)],
+        "g500_msbfs_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -253,6 +279,79 @@ class Bfs:
     def destroy(self):
         if self.h:
             _check(load().g500_bfs_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class MultiBfs:
+    """Batched bit-parallel BFS: up to 256 roots per launch, uint8 levels."""
+
+    def __init__(self, graph, max_roots=MSBFS_MAX_ROOTS):
+        self.graph = graph
+        h = C.c_void_p()
+        _check(load().g500_msbfs_create(graph.h, int(max_roots), C.byref(h)))
+        self.h = h
+        self.nroots = 0
+
+    def tuning(self):
+        t = MsbfsTuning()
+        _check(load().g500_msbfs_get_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in MsbfsTuning._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = MsbfsTuning(**cur)
+        _check(load().g500_msbfs_set_tuning(self.h, C.byref(t)))
+
+    def launch(self, roots):
+        self._roots = np.ascontiguousarray(roots, np.int64)
+        _check(load().g500_msbfs_launch(self.h, _p(self._roots, C.c_int64), len(self._roots)))
+        self.nroots = len(self._roots)
+
+    def done(self):
+        d = C.c_int()
+        _check(load().g500_msbfs_query(self.h, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self):
+        _check(load().g500_msbfs_wait(self.h))
+
+    def run(self, roots):
+        r = np.ascontiguousarray(roots, np.int64)
+        _check(load().g500_msbfs_run(self.h, _p(r, C.c_int64), len(r)))
+        self.nroots = len(r)
+
+    def levels(self, index):
+        out = np.empty(self.graph.info.n, np.uint8)
+        _check(load().g500_msbfs_get_levels(self.h, int(index), _p(out, C.c_uint8)))
+        return out
+
+    def device_levels(self):
+        p, stride = C.c_void_p(), C.c_int64()
+        _check(load().g500_msbfs_device_levels(self.h, C.byref(p), C.byref(stride)))
+        return p.value, stride.value
+
+    def te(self):
+        out = np.empty(max(self.nroots, 1), np.int64)
+        _check(load().g500_msbfs_get_te(self.h, _p(out, C.c_int64)))
+        return out[:self.nroots]
+
+    def stats(self):
+        st = MsbfsStats()
+        _check(load().g500_msbfs_get_stats(self.h, C.byref(st)))
+        return dict(nroots=st.nroots, nlevels=st.nlevels, gpu_ms=st.gpu_ms,
+                    discovered=st.discovered,
+                    lane_discoveries=list(st.lane_discoveries[:min(st.nlevels, 64)]))
+
+    def destroy(self):
+        if self.h:
+            _check(load().g500_msbfs_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -38,6 +38,7 @@ struct GraphView {
   int64_t num_global_edges_ = 0;   // deduplicated directed CSR edges
   int64_t num_global_verts_ = 0;   // vertices with at least one edge
   int64_t max_used_vertex_ = 0;
+  int reorder_bits_ = 0;           // printed under SMALL_REORDER_BIT (mpi/main.cc:691-694)
 
   int64_t pred_size() const { return num_orig_local_verts_; }
   int log_orig_global_verts() const { return log_orig_global_verts_; }
@@ -181,6 +182,7 @@ class BfsOnMI355X {
     graph_.num_global_edges_ = inf.m;
     graph_.num_global_verts_ = inf.n_c;
     graph_.max_used_vertex_ = inf.max_used_vertex;
+    graph_.reorder_bits_ = inf.log_n;  // new IDs are log_n-bit wide
   }
 
   int device_;

@@ -214,6 +214,70 @@ int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st,
 /* BfsBase::end_bfs (mpi/bfs.hpp:165): frees the engine. */
 int g500_bfs_destroy(g500_bfs* b);
 
+/* ---- Batched bit-parallel multi-source BFS -------------------------------
+ * No counterpart in the reference, which runs one root per run_bfs
+ * (mpi/bfs.hpp:3214).  Up to G500_MSBFS_MAX_ROOTS roots traverse the graph
+ * together as 64-bit lane masks per vertex.  The result per root is its uint8
+ * level array by original ID (0xFF = unreached), equal to the depths that
+ * g500_bfs_run finds for that root.  Graphs built with
+ * G500_CONSTRUCT_COREBFS are not supported (their CSR holds the core only). */
+#define G500_MSBFS_MAX_ROOTS 256
+typedef struct g500_msbfs g500_msbfs;
+
+/* Work split of the level sweeps (new IDs are degree-descending, so each
+ * degree bin is a contiguous range).  Vertices with degree >= heavy_degree are
+ * cut into work items of heavy_chunk edges, one wave per item; those with
+ * degree >= medium_degree get medium_group lanes each; the rest one lane
+ * each.  A level runs top-down (push from the frontier) while its frontier's
+ * summed degree is below m / td_alpha, else bottom-up: while fewer than
+ * stream_frac of all (root, vertex) pairs have been reached, as one balanced
+ * stream over every edge (early exit rarely pays then), later per vertex with
+ * early exit.
+ * blocks_per_cu sizes the grid of the persistent sweep kernels; use_graph = 1
+ * replays the whole batch as one captured HIP graph. */
+typedef struct g500_msbfs_tuning {
+  int64_t heavy_degree;
+  int64_t medium_degree;
+  int32_t medium_group;  /* 2, 4, 8, 16 or 32 */
+  int32_t blocks_per_cu; /* 1..32 */
+  int32_t use_graph;     /* 0 or 1 */
+  int32_t heavy_chunk;   /* 256..65536, multiple of 256 */
+  double td_alpha;       /* > 0 */
+  double stream_frac;    /* 0..1; 0 = never stream */
+} g500_msbfs_tuning;
+
+typedef struct g500_msbfs_stats {
+  int32_t nroots;
+  int32_t nlevels;        /* levels with a non-empty frontier, incl. level 0 */
+  double gpu_ms;          /* device time of the last batch (events on its stream) */
+  int64_t discovered;     /* (root, vertex) pairs reached, summed over roots */
+  int64_t lane_discoveries[64]; /* per level (first 64 levels) */
+} g500_msbfs_stats;
+
+/* Allocates the engine for batches of up to max_roots roots (1..256). */
+int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out);
+int g500_msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning* t);
+int g500_msbfs_get_tuning(g500_msbfs* b, g500_msbfs_tuning* t);
+/* Enqueues a batch (roots by original ID, 1..max_roots of them, duplicates
+ * allowed) on the engine's stream and returns without waiting. */
+int g500_msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots);
+/* *done = 1 when the launched batch has finished on the device. */
+int g500_msbfs_query(g500_msbfs* b, int* done);
+/* Waits for the launched batch and completes it (levels beyond the captured
+ * level count run here).  Results stay on the device until read. */
+int g500_msbfs_wait(g500_msbfs* b);
+/* g500_msbfs_launch + g500_msbfs_wait. */
+int g500_msbfs_run(g500_msbfs* b, const int64_t* roots, int nroots);
+/* Level array of root `index` of the last batch: N = 2^log_n bytes. */
+int g500_msbfs_get_levels(g500_msbfs* b, int index, uint8_t* host_levels);
+/* Device address of the level arrays ([nroots][N] bytes, row stride N). */
+int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride);
+/* Traversed edges per root (input edges whose source the root reaches, the
+ * validator's TE, mpi/validate.hpp:840-872); te holds nroots entries. */
+int g500_msbfs_get_te(g500_msbfs* b, int64_t* te);
+int g500_msbfs_get_stats(g500_msbfs* b, g500_msbfs_stats* st);
+int g500_msbfs_destroy(g500_msbfs* b);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* g500_last_error(void);
 
