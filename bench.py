@@ -9,7 +9,14 @@ device during the timed BFS (get_pred is outside the timer, as in the
 reference).  Every timed root is validated afterwards on the device and its
 traversed-edge count (TE) is the TEPS numerator.
 
-Prints one JSON line (see DESIGN.md for the roofline yardstick).
+Two secondary measurements ride on the same line: "batched" (the
+bit-parallel multi-source BFS on the same graph, 64/128/256 roots per launch,
+traversed edges/s and its ratio to the plain form) and "serving" (continuous
+batching of BFS queries over four resident graphs, SCALE 20/23/25/27, open
+Poisson arrivals; p50/p99 latency and queries/s beside a one-query-per-launch
+baseline).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -40,6 +47,11 @@ def parse():
     ap.add_argument("--no-validate", action="store_true",
                     help="skip per-root validation (TE then comes from pf_nedge)")
     ap.add_argument("--levels", action="store_true", help="print per-level trace of one root")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched multi-source BFS measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-queries", type=int, default=4000)
+    ap.add_argument("--serve-qps", type=float, default=2000.0)
     return ap.parse_args()
 
 
@@ -108,6 +120,80 @@ def cpu_baseline(graph, roots, te_by_root, budget_s, args):
             "sample": f"{len(times)} root(s) of the same SCALE {args.scale} graph, serial C "
                       f"direction-optimizing BFS (oracle/g500_oracle.c orc_do_bfs), "
                       f"{sum(times):.1f} s"}
+
+
+def batched(g500bfs, graph, bfs, args, plain_teps):
+    """Batched bit-parallel BFS on the same graph: tuned on one root set,
+    timed on another, levels spot-checked against the plain engine."""
+    mb = g500bfs.MultiBfs(graph, max_roots=256)
+    tune_roots = graph.find_roots(256, r1=101)
+    tuning, tune_ms = mb.autotune(tune_roots, rounds=1)
+    log(f"batched tuning {tuning} ({tune_ms:.2f} ms per 256-root batch)")
+    roots = graph.find_roots(256, r1=0)
+    out = {"tuning": tuning}
+    for nr in (64, 128, 256):
+        r = roots[:nr]
+        mb.run(r)
+        host, dev = [], []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            mb.run(r)
+            host.append(time.perf_counter() - t0)
+            dev.append(mb.stats()["gpu_ms"])
+        te = mb.te()
+        t = float(np.median(host))
+        exact = True
+        for i in (0, nr // 2, nr - 1):
+            bfs.run(int(r[i]), args.edgefactor, args.alpha, args.beta)
+            l16 = bfs.levels()
+            exact &= bool(np.array_equal(mb.levels(i),
+                                         np.where(l16 == 0xFFFF, 0xFF, l16).astype(np.uint8)))
+        teps = float(te.sum()) / t
+        out[str(nr)] = {"ms_per_batch": round(t * 1e3, 3),
+                        "gpu_ms_per_batch": round(float(np.median(dev)), 3),
+                        "roots_per_s": round(nr / t, 1),
+                        "traversed_edges_per_s": round(teps, 1),
+                        "x_plain_teps": round(teps / plain_teps, 2),
+                        "levels_bit_exact_spot_check": exact}
+        log(f"batched {nr}: {t * 1e3:.2f} ms, {teps / 1e12:.3f} TTEPS, "
+            f"{teps / plain_teps:.2f}x plain, exact={exact}")
+    mb.destroy()
+    return out
+
+
+def serving(g500bfs, args):
+    """Continuous-batching serving loop over four resident Kronecker graphs
+    (SCALE 20, 23, 25, 27; seeds (2+i, 3+i)), open Poisson arrivals."""
+    import serve
+    graphs, pools = [], []
+    for i, scale in enumerate((20, 23, 25, 27)):
+        g = g500bfs.Graph.generate(scale, 16, 2 + i, 3 + i).construct()
+        graphs.append(g)
+        pools.append(g.find_roots(512))
+    srv = serve.Server(graphs, max_batch=256)
+    srv.prewarm(1, pools)
+    qs = serve.poisson_workload(graphs, pools, args.serve_queries, args.serve_qps, seed=7)
+    t0 = time.perf_counter()
+    done = serve.run_open_loop(srv, qs)
+    wall = time.perf_counter() - t0
+    res = serve.latency_summary(done)
+    res.update({"offered_qps": args.serve_qps, "wall_s": round(wall, 3),
+                "batches": srv.batches,
+                "engines": [len(es) for es in srv.engines],
+                "engine_gib": round(srv.used / 2**30, 1)})
+    srv.close()
+    sample = [serve.Query(q.qid, q.gid, q.root, q.targets) for q in done[:200]]
+    base = serve.one_per_launch(graphs, sample)
+    same = all(a.te == b.te and np.array_equal(a.levels, b.levels)
+               for a, b in zip(done[:200], sample))
+    res.update({"one_query_per_launch": base, "answers_equal_alone": bool(same),
+                "graphs": "Kronecker SCALE 20/23/25/27, edgefactor 16, seeds (2,3)..(5,6)"})
+    for k in ("p50_ms", "p99_ms", "mean_ms", "qps"):
+        res[k] = round(res[k], 3)
+    log(f"serving: {res}")
+    for g in graphs:
+        g.free()
+    return res
 
 
 def main():
@@ -210,6 +296,15 @@ def main():
         except Exception as e:  # the baseline is a report, never the product
             log(f"cpu baseline failed: {e}")
 
+    bat = None
+    if not args.no_batched:
+        bat = batched(g500bfs, graph, bfs, args, teps_hm)
+    bfs.destroy()
+    graph.free()
+    srv = None
+    if not args.no_serve:
+        srv = serving(g500bfs, args)
+
     out = {
         "metric": "harmonic-mean GTEPS @SCALE 26, 1 GPU",
         "value": round(teps_hm / 1e9, 3),
@@ -248,6 +343,8 @@ def main():
             "mean_bfs_gpu_ms": round(mean_gpu_s * 1e3, 4),
         },
         "cpu_baseline": cpu,
+        "batched": bat,
+        "serving": srv,
     }
     print(json.dumps(out), flush=True)
 

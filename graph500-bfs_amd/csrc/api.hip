@@ -30,6 +30,10 @@ thread_local std::string t_last_error;
 
 template <class F>
 int guard(F&& f) {
+  // launch checks read hipGetLastError: drop any error an earlier call left
+  // behind (e.g. a best-effort call in a destructor) so that it is not
+  // reported against this entry point
+  (void)hipGetLastError();
   try {
     f();
     return G500_OK;
@@ -395,6 +399,27 @@ int g500_bfs_destroy(g500_bfs* b) {
   });
 }
 
+int g500_msbfs_footprint(const g500_graph* g, int max_roots, int64_t* bytes) {
+  return guard([&] {
+    need_graph(g, true);
+    if (!bytes) g500::fail(G500_ERR_ARG, "null output pointer");
+    if (max_roots < 1 || max_roots > G500_MSBFS_MAX_ROOTS)
+      g500::fail(G500_ERR_ARG, "max_roots %d outside [1, %d]", max_roots, G500_MSBFS_MAX_ROOTS);
+    *bytes = g500::msbfs_footprint(g, max_roots);
+  });
+}
+
+int g500_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+  return guard([&] {
+    if (!free_bytes || !total_bytes) g500::fail(G500_ERR_ARG, "null output pointer");
+    use_device(device);
+    size_t f = 0, t = 0;
+    G500_CHECK_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+  });
+}
+
 int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out) {
   return guard([&] {
     need_graph(g, true);
@@ -482,6 +507,27 @@ int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride)
     if (!dev_ptr || !row_stride) g500::fail(G500_ERR_ARG, "null output pointer");
     *dev_ptr = b->out.get();
     *row_stride = b->g->n;
+  });
+}
+
+int g500_msbfs_gather_levels(g500_msbfs* b, const int64_t* pairs, int64_t count,
+                             uint8_t* levels) {
+  return guard([&] {
+    need_result(b);
+    if (count < 0 || (count > 0 && (!pairs || !levels)))
+      g500::fail(G500_ERR_ARG, "bad pair list");
+    g500::msbfs_gather(b, pairs, count, levels);
+  });
+}
+
+int g500_msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int rounds,
+                        g500_msbfs_tuning* best, double* best_ms) {
+  return guard([&] {
+    need_msbfs(b);
+    if (b->in_flight) g500::fail(G500_ERR_STATE, "batch in flight");
+    if (rounds < 1 || rounds > 8) g500::fail(G500_ERR_ARG, "rounds %d outside [1, 8]", rounds);
+    const double ms = g500::msbfs_autotune(b, roots, nroots, rounds, best);
+    if (best_ms) *best_ms = ms;
   });
 }
 

@@ -37,7 +37,7 @@ constexpr int kT = 256;
 constexpr int kWavesPerBlock = kT / kWave;
 constexpr int kMaxLevels = 255;    // uint8 levels 0..254, 0xFF = unreached
 // level buffers F_0..F_255: the last one only proves that depth 255 is empty
-constexpr int kInitialLevels = 16;
+constexpr int kInitialLevels = 12;  // Kronecker graphs need 8-10; deeper ones grow
 
 __device__ inline uint64_t pad_mask(int k, int nroots) {
   const int nvalid = min(max(nroots - 64 * k, 0), 64);
@@ -826,6 +826,16 @@ __global__ __launch_bounds__(kT) void k_ms_output_tile(const int64_t* __restrict
   }
 }
 
+// levels[i] = out[root_index[i]][vertex[i]] (0xFF for vertices outside [0, n))
+__global__ void k_ms_gather(const uint8_t* __restrict__ out, int64_t n,
+                            const int64_t* __restrict__ pairs, int64_t count,
+                            uint8_t* __restrict__ levels) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t r = pairs[2 * i], v = pairs[2 * i + 1];
+  levels[i] = (v >= 0 && v < n) ? out[r * n + v] : (uint8_t)0xFF;
+}
+
 // Roots without edges reach only themselves.
 __global__ void k_ms_isolated(const int64_t* __restrict__ d_roots,
                               const uint32_t* __restrict__ o2n, int64_t n, int64_t n_c,
@@ -1147,6 +1157,19 @@ void update_bins(g500_msbfs* b) {
 
 }  // namespace
 
+int64_t msbfs_footprint(const g500_graph* g, int max_roots) {
+  const int64_t kmax = ceil_div(max_roots, 64);
+  const int64_t nc = std::max<int64_t>(1, g->n_c);
+  const int64_t bw = std::max<int64_t>(1, ceil_div(g->n_c, 64));
+  int64_t bytes = nc * kmax * 8;                      // seen
+  bytes += (int64_t)kInitialLevels * nc * kmax * 8;   // level frontiers
+  bytes += (int64_t)max_roots * g->n;                 // level arrays
+  bytes += g->n * 4;                                  // cnt0
+  bytes += 2 * bw * 8;                                // frontier bitmaps
+  bytes += 4 * (kMaxLevels + 2) * 8 + 8 * 1024;       // counters, roots, te, tables
+  return bytes;
+}
+
 void msbfs_init(g500_msbfs* b, int max_roots) {
   g500_graph* g = b->g;
   b->max_roots = max_roots;
@@ -1325,6 +1348,79 @@ void msbfs_wait(g500_msbfs* b) {
     if (l < 64) st.lane_discoveries[l] = (int64_t)b->h_cnt[l];
   }
   b->have_result = true;
+}
+
+void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels) {
+  if (count <= 0) return;
+  for (int64_t i = 0; i < count; ++i)
+    if (pairs[2 * i] < 0 || pairs[2 * i] >= b->nroots)
+      fail(G500_ERR_ARG, "pair %lld: root index %lld outside [0, %d)", (long long)i,
+           (long long)pairs[2 * i], b->nroots);
+  DevBuf<int64_t> dp;
+  DevBuf<uint8_t> dl;
+  dp.alloc((size_t)count * 2);
+  dl.alloc((size_t)count);
+  hipStream_t s = b->stream;
+  G500_CHECK_HIP(hipMemcpyAsync(dp.get(), pairs, (size_t)count * 2 * sizeof(int64_t),
+                                hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ms_gather, dim3((unsigned)ceil_div(count, kT)), dim3(kT), 0, s,
+                     b->out.get(), b->g->n, dp.get(), count, dl.get());
+  G500_CHECK_LAUNCH();
+  G500_CHECK_HIP(hipMemcpyAsync(levels, dl.get(), (size_t)count, hipMemcpyDeviceToHost, s));
+  G500_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+// Coordinate search over the work split and the direction rule: each
+// parameter in turn takes every candidate value with the others fixed, the
+// fastest batch (device time, best of two runs) is kept; `rounds` passes.
+double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int rounds,
+                      g500_msbfs_tuning* best_out) {
+  g500_msbfs_tuning best = b->tuning;
+  auto measure = [&](const g500_msbfs_tuning& t) {
+    msbfs_set_tuning(b, t);
+    double ms = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {  // the first run also captures the graph
+      msbfs_launch(b, roots, nroots);
+      msbfs_wait(b);
+      if (rep > 0) ms = std::min(ms, b->stats.gpu_ms);
+    }
+    return ms;
+  };
+  double best_ms = measure(best);
+  const int64_t heavy[] = {512, 1024, 2048, 4096, 8192, 16384};
+  const int64_t medium[] = {8, 16, 32, 64, 128, 256};
+  const int32_t group[] = {4, 8, 16, 32};
+  const int32_t bpc[] = {2, 4, 8, 16};
+  const int32_t chunk[] = {1024, 2048, 4096, 8192, 16384};
+  const double alpha[] = {1, 2, 4, 8, 16, 32, 64};
+  const double sfrac[] = {0.0, 0.05, 0.2};
+  for (int round = 0; round < rounds; ++round) {
+    for (int p = 0; p < 7; ++p) {
+      const int nc = p == 0 ? 6 : p == 1 ? 6 : p == 2 ? 4 : p == 3 ? 4 : p == 4 ? 5 : p == 5 ? 7 : 3;
+      for (int c = 0; c < nc; ++c) {
+        g500_msbfs_tuning t = best;
+        switch (p) {
+          case 0: t.heavy_degree = heavy[c]; break;
+          case 1: t.medium_degree = medium[c]; break;
+          case 2: t.medium_group = group[c]; break;
+          case 3: t.blocks_per_cu = bpc[c]; break;
+          case 4: t.heavy_chunk = chunk[c]; break;
+          case 5: t.td_alpha = alpha[c]; break;
+          default: t.stream_frac = sfrac[c]; break;
+        }
+        if (t.medium_degree > t.heavy_degree) continue;
+        if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;
+        const double ms = measure(t);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+    }
+  }
+  msbfs_set_tuning(b, best);
+  if (best_out) *best_out = best;
+  return best_ms;
 }
 
 }  // namespace g500

@@ -61,8 +61,13 @@ struct g500_msbfs {
 
 namespace g500 {
 void msbfs_init(g500_msbfs* b, int max_roots);
+// device bytes msbfs_init allocates (before the heavy work-item tables)
+int64_t msbfs_footprint(const g500_graph* g, int max_roots);
 void msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning& t);
 void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots);
 bool msbfs_query(g500_msbfs* b);
 void msbfs_wait(g500_msbfs* b);
+void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels);
+double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int rounds,
+                      g500_msbfs_tuning* best);
 }  // namespace g500

@@ -94,6 +94,8 @@ def load():
         "g500_bfs_get_stats": [vp, C.POINTER(RunStats), C.POINTER(LevelTrace), C.c_int],
         "g500_bfs_destroy": [vp],
         "g500_msbfs_create": [vp, C.c_int, C.POINTER(vp)],
+        "g500_msbfs_footprint": [vp, C.c_int, i64p],
+        "g500_device_memory": [C.c_int, i64p, i64p],
         "g500_msbfs_set_tuning": [vp, C.POINTER(MsbfsTuning)],
         "g500_msbfs_get_tuning": [vp, C.POINTER(MsbfsTuning)],
         "g500_msbfs_launch": [vp, i64p, C.c_int],
@@ -103,6 +105,9 @@ def load():
         "g500_msbfs_get_levels": [vp, C.c_int, C.POINTER(C.c_uint8)],
         "g500_msbfs_device_levels": [vp, C.POINTER(vp), i64p],
         "g500_msbfs_get_te": [vp, i64p],
+        "g500_msbfs_gather_levels": [vp, i64p, i64, C.POINTER(C.c_uint8)],
+        "g500_msbfs_autotune": [vp, i64p, C.c_int, C.c_int, C.POINTER(MsbfsTuning),
+                                C.POINTER(C.c_double)],
         "g500_msbfs_get_stats": [vp, C.POINTER(MsbfsStats)],
         "g500_msbfs_destroy": [vp],
     }
@@ -114,6 +119,19 @@ def load():
     L.g500_last_error.restype = C.c_char_p
     _lib = L
     return L
+
+
+def device_memory(device=0):
+    """(free, total) device bytes."""
+    f, t = C.c_int64(), C.c_int64()
+    _check(load().g500_device_memory(device, C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+def msbfs_footprint(graph, max_roots):
+    b = C.c_int64()
+    _check(load().g500_msbfs_footprint(graph.h, int(max_roots), C.byref(b)))
+    return b.value
 
 
 def make_random_numbers(n, seed1, seed2, position):
@@ -336,6 +354,25 @@ class MultiBfs:
         p, stride = C.c_void_p(), C.c_int64()
         _check(load().g500_msbfs_device_levels(self.h, C.byref(p), C.byref(stride)))
         return p.value, stride.value
+
+    def gather(self, root_index, vertices):
+        """Levels of `vertices` (original IDs) for batch roots `root_index`."""
+        ri = np.broadcast_to(np.asarray(root_index, np.int64), np.shape(vertices))
+        pairs = np.ascontiguousarray(np.stack([ri, np.asarray(vertices, np.int64)], axis=-1)
+                                     .reshape(-1, 2))
+        out = np.empty(len(pairs), np.uint8)
+        if len(pairs):
+            _check(load().g500_msbfs_gather_levels(self.h, _p(pairs, C.c_int64), len(pairs),
+                                                   _p(out, C.c_uint8)))
+        return out
+
+    def autotune(self, roots, rounds=1):
+        r = np.ascontiguousarray(roots, np.int64)
+        t, ms = MsbfsTuning(), C.c_double()
+        _check(load().g500_msbfs_autotune(self.h, _p(r, C.c_int64), len(r), int(rounds),
+                                          C.byref(t), C.byref(ms)))
+        self.nroots = len(r)
+        return {f: getattr(t, f) for f, _ in MsbfsTuning._fields_}, ms.value
 
     def te(self):
         out = np.empty(max(self.nroots, 1), np.int64)

@@ -254,6 +254,11 @@ typedef struct g500_msbfs_stats {
   int64_t lane_discoveries[64]; /* per level (first 64 levels) */
 } g500_msbfs_stats;
 
+/* Device bytes g500_msbfs_create(g, max_roots) will allocate (admission
+ * control of a serving loop checks it against g500_device_memory). */
+int g500_msbfs_footprint(const g500_graph* g, int max_roots, int64_t* bytes);
+/* Free and total device memory (hipMemGetInfo). */
+int g500_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
 /* Allocates the engine for batches of up to max_roots roots (1..256). */
 int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out);
 int g500_msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning* t);
@@ -272,6 +277,16 @@ int g500_msbfs_run(g500_msbfs* b, const int64_t* roots, int nroots);
 int g500_msbfs_get_levels(g500_msbfs* b, int index, uint8_t* host_levels);
 /* Device address of the level arrays ([nroots][N] bytes, row stride N). */
 int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride);
+/* Point lookups into the last batch: levels[i] = level of vertex pairs[2i+1]
+ * (original ID) for the root at batch index pairs[2i] (0xFF = unreached or
+ * vertex out of range). */
+int g500_msbfs_gather_levels(g500_msbfs* b, const int64_t* pairs, int64_t count,
+                             uint8_t* levels);
+/* Picks the tuning fields by timing batches of `roots` (coordinate search,
+ * `rounds` passes of 1..8); leaves the best tuning set and returns it with its
+ * device time per batch. */
+int g500_msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int rounds,
+                        g500_msbfs_tuning* best, double* best_ms);
 /* Traversed edges per root (input edges whose source the root reaches, the
  * validator's TE, mpi/validate.hpp:840-872); te holds nroots entries. */
 int g500_msbfs_get_te(g500_msbfs* b, int64_t* te);

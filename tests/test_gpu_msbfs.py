@@ -212,3 +212,32 @@ def test_two_vertex_graph(G):
     assert mb.levels(1).tolist() == [1, 0]
     assert mb.levels(2).tolist() == [0, 1]
     assert list(mb.te()) == [1, 1, 1]
+
+
+def test_gather_levels(G, s12):
+    print("\n[msbfs] gather", flush=True)
+    g, og = s12
+    roots = _mixed_roots(g, og, 70, seed=11)
+    mb = G.MultiBfs(g)
+    mb.run(roots)
+    rng = np.random.default_rng(5)
+    ri = rng.integers(0, 70, 500)
+    vs = rng.integers(-3, g.info.n + 3, 500)
+    got = mb.gather(ri, vs)
+    for i, r, v in zip(range(500), ri, vs):
+        exp = 0xFF if v < 0 or v >= g.info.n else mb.levels(int(r))[v]
+        assert got[i] == exp
+    with pytest.raises(G.G500Error):
+        mb.gather([70], [0])
+
+
+def test_autotune_keeps_levels(G, s12):
+    print("\n[msbfs] autotune", flush=True)
+    g, og = s12
+    roots = _mixed_roots(g, og, 64, seed=13)
+    mb = G.MultiBfs(g, max_roots=64)
+    best, ms = mb.autotune(roots, rounds=1)
+    assert ms > 0 and best == mb.tuning()
+    mb.run(roots)
+    for i in (0, 2, 3, 40, 63):
+        assert np.array_equal(mb.levels(i), _u8(og.bfs(int(roots[i]))))
