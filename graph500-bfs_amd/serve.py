@@ -41,9 +41,9 @@ class Query:
 
 
 class _Engine:
-    def __init__(self, graph, max_batch):
-        self.mb = G.MultiBfs(graph, max_roots=max_batch)
-        self.batch = None  # queries of the batch in flight
+    def __init__(self, mb):
+        self.mb = mb        # a g500bfs.MultiBfs (or a stand-in with its methods)
+        self.batch = None   # queries of the batch in flight
 
 
 class Server:
@@ -55,7 +55,7 @@ class Server:
     """
 
     def __init__(self, graphs, max_batch=256, hbm_budget=None, max_engines_per_graph=4,
-                 device=0):
+                 device=0, engine_factory=None, footprint=None):
         self.graphs = list(graphs)
         self.max_batch = int(max_batch)
         if hbm_budget is None:
@@ -64,7 +64,9 @@ class Server:
         self.budget = int(hbm_budget)
         self.used = 0
         self.max_engines = int(max_engines_per_graph)
-        self.footprint = [G.msbfs_footprint(g, self.max_batch) for g in self.graphs]
+        self.new_engine = engine_factory or (lambda g, r: G.MultiBfs(g, max_roots=r))
+        self.footprint = list(footprint) if footprint is not None else \
+            [G.msbfs_footprint(g, self.max_batch) for g in self.graphs]
         self.engines = [[] for _ in self.graphs]
         self.queues = [collections.deque() for _ in self.graphs]
         self.done = []
@@ -91,7 +93,7 @@ class Server:
                           if g != gid and not self.engines[g])
         if len(self.engines[gid]) < self.max_engines and \
                 self.used + self.footprint[gid] + reserve <= self.budget:
-            e = _Engine(self.graphs[gid], self.max_batch)
+            e = _Engine(self.new_engine(self.graphs[gid], self.max_batch))
             self.engines[gid].append(e)
             self.used += self.footprint[gid]
             return e
@@ -151,7 +153,7 @@ class Server:
                 if len(self.engines[gid]) >= self.max_engines or \
                         self.used + self.footprint[gid] > self.budget:
                     break
-                e = _Engine(g, self.max_batch)
+                e = _Engine(self.new_engine(g, self.max_batch))
                 self.engines[gid].append(e)
                 self.used += self.footprint[gid]
             pool = root_pools[gid] if root_pools is not None else g.find_roots(1)
