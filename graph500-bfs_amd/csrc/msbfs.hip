@@ -170,10 +170,13 @@ struct MsLevel {
   const int64_t* d_roots;  // [0] = batch size
   double stream_frac;
   int64_t n_c;
+  int64_t hot;             // dense_algo 2: vertices below this are pulled
+  int dense_algo;          // dense levels: 1 streaming pull, 2 hub pull + push
   int L;
 };
 
 enum { kModeTopDown = 0, kModeStream = 1, kModePull = 2 };
+enum { kDenseStream = 1, kDenseHybrid = 2 };
 
 __device__ inline bool ms_active(const MsLevel& a) { return a.cnt[a.L] != 0; }
 __device__ inline bool ms_top_down(const MsLevel& a, int l) { return a.fe[l] < a.thr; }
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m) {
   __shared__ unsigned long long acc_l[kTile * K];
   __shared__ uint8_t full_l[kTile];
   __shared__ int64_t s_v0, s_wn;
-  if (!ms_active(a) || ms_mode(a, a.L) != kModeStream) return;
+  if (!ms_active(a) || a.dense_algo != kDenseStream || ms_mode(a, a.L) != kModeStream) return;
   const int t = threadIdx.x;
   const int64_t ntiles = ceil_div(m, kTile);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -430,6 +433,171 @@ __global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m) {
       }
     }
     __syncthreads();
+  }
+}
+
+// Dense level, hybrid form.  Rows are sorted by target ID, so the targets
+// below the hub cut-off `hot` form a prefix of every row: each vertex ORs the
+// frontier lanes of that prefix (their lane words are few and stay cached),
+// then every frontier vertex at or above the cut-off pushes its lane word
+// along its row (mostly into hubs' words, also cached).  Each directed edge
+// v <- u is taken once: by v's pull when u < hot, by u's push otherwise.
+// F_{L+1} collects the unmasked lanes; k_ms_commit masks them with seen.
+template <int K, int G, int U>
+__global__ __launch_bounds__(kT) void k_ms_hyb_pull(MsLevel a, int64_t vb, int64_t ve) {
+  if (!ms_active(a) || a.dense_algo != kDenseHybrid || ms_mode(a, a.L) != kModeStream) return;
+  const int lig = threadIdx.x % G;
+  const int64_t gid = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
+  const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
+  const uint32_t hot = (uint32_t)a.hot;
+  for (int64_t v = vb + gid; v < ve; v += ngroups) {
+    uint64_t s[K], f[K];
+    load_lanes<K>(a.seen + v * K, s);
+    load_lanes<K>(a.fl + v * K, f);
+    bool full = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      s[k] |= f[k];
+      full = full && (s[k] == ~0ull);
+    }
+    if (full) continue;
+    const int64_t beg = a.row_off[v], end = a.row_off[v + 1];
+    uint64_t acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0;
+    for (int64_t base = beg; base < end; base += (int64_t)G * U) {
+      uint32_t u[U];
+      bool past = false;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int64_t e = base + (int64_t)j * G + lig;
+        u[j] = e < end ? a.col[e] : 0xFFFFFFFFu;
+        past = past || (e < end && u[j] >= hot);
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (u[j] < hot && in_bitmap(a.fb, u[j])) {
+          uint64_t x[K];
+          load_lanes<K>(a.fl + (int64_t)u[j] * K, x);
+#pragma unroll
+          for (int k = 0; k < K; ++k) acc[k] |= x[k];
+        }
+      }
+      if constexpr (G > 1) {
+        int pi = past ? 1 : 0;  // the prefix ends inside this round (group-wide)
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) pi |= __shfl_xor(pi, o, kWave);
+        past = pi != 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int o = G / 2; o > 0; o >>= 1) acc[k] |= __shfl_xor(acc[k], o, kWave);
+      }
+      if (past) break;
+    }
+    if (lig == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (acc[k]) a.fn[v * K + k] = acc[k];
+    }
+  }
+}
+
+// Heavy rows of the hybrid pull, one wave per work item of `chunk` edges.
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_hyb_pull_heavy(MsLevel a,
+                                                         const uint64_t* __restrict__ item_start,
+                                                         const uint32_t* __restrict__ item_vertex,
+                                                         int64_t n_items, int chunk) {
+  if (!ms_active(a) || a.dense_algo != kDenseHybrid || ms_mode(a, a.L) != kModeStream) return;
+  const int lane = lane_id();
+  const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+  const uint32_t hot = (uint32_t)a.hot;
+  for (int64_t it = wid; it < n_items; it += nw) {
+    const int64_t v = item_vertex[it];
+    const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
+    const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
+    if (a.col[beg] >= hot) continue;  // the row's hub prefix ended earlier
+    uint64_t s[K], f[K];
+    load_lanes<K>(a.seen + v * K, s);
+    load_lanes<K>(a.fl + v * K, f);
+    bool full = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) full = full && ((s[k] | f[k]) == ~0ull);
+    if (full) continue;
+    uint64_t acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0;
+    for (int64_t base = beg; base < end; base += kWave) {
+      const int64_t e = base + lane;
+      const uint32_t u = e < end ? a.col[e] : 0xFFFFFFFFu;
+      if (u < hot && in_bitmap(a.fb, u)) {
+        uint64_t x[K];
+        load_lanes<K>(a.fl + (int64_t)u * K, x);
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] |= x[k];
+      }
+      if (__any(e < end && u >= hot)) break;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc[k] |= __shfl_xor(acc[k], o, kWave);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (acc[k]) atomicOr((unsigned long long*)&a.fn[v * K + k], (unsigned long long)acc[k]);
+    }
+  }
+}
+
+template <int K>
+__device__ inline void hyb_push_one(const MsLevel& a, uint32_t w, const uint64_t (&f)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    unsigned long long* p = (unsigned long long*)&a.fn[(int64_t)w * K + k];
+    // lanes only ever get added, so a stale read can only cause an extra atomic
+    const uint64_t m = f[k] & ~(uint64_t)__atomic_load_n(p, __ATOMIC_RELAXED);
+    if (m) atomicOr(p, (unsigned long long)m);
+  }
+}
+
+// Push side of the hybrid: frontier vertices in [max(vb, hot), ve), G lanes each.
+template <int K, int G>
+__global__ __launch_bounds__(kT) void k_ms_hyb_push(MsLevel a, int64_t vb, int64_t ve) {
+  if (!ms_active(a) || a.dense_algo != kDenseHybrid || ms_mode(a, a.L) != kModeStream) return;
+  const int lig = threadIdx.x % G;
+  const int64_t gid = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
+  const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
+  for (int64_t v = max(vb, a.hot) + gid; v < ve; v += ngroups) {
+    if (!in_bitmap(a.fb, v)) continue;
+    uint64_t f[K];
+    load_lanes<K>(a.fl + v * K, f);
+    const int64_t end = a.row_off[v + 1];
+    for (int64_t e = a.row_off[v] + lig; e < end; e += G) hyb_push_one<K>(a, a.col[e], f);
+  }
+}
+
+// Push side for heavy rows at or above the cut-off: items [first_item, n_items).
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_hyb_push_heavy(MsLevel a,
+                                                         const uint64_t* __restrict__ item_start,
+                                                         const uint32_t* __restrict__ item_vertex,
+                                                         int64_t first_item, int64_t n_items,
+                                                         int chunk) {
+  if (!ms_active(a) || a.dense_algo != kDenseHybrid || ms_mode(a, a.L) != kModeStream) return;
+  const int lane = lane_id();
+  const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+  for (int64_t it = first_item + wid; it < n_items; it += nw) {
+    const int64_t v = item_vertex[it];
+    if (!in_bitmap(a.fb, v)) continue;
+    uint64_t f[K];
+    load_lanes<K>(a.fl + v * K, f);
+    const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
+    const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
+    for (int64_t e = beg + lane; e < end; e += kWave) hyb_push_one<K>(a, a.col[e], f);
   }
 }
 
@@ -937,6 +1105,8 @@ MsLevel level_args(const g500_msbfs* b, int L) {
   a.thr = b->td_threshold;
   a.d_roots = b->roots.get();
   a.stream_frac = b->tuning.stream_frac;
+  a.dense_algo = b->tuning.dense_algo;
+  a.hot = 0;  // set per lane-word count by level_kernels
   a.n_c = g->n_c;
   a.L = L;
   return a;
@@ -959,9 +1129,29 @@ void launch_bu(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve, hi
 
 // One level L -> L+1: preparation, then the top-down and the bottom-up
 // kernels of every bin (the ones of the other direction return at once).
+template <int K, int G>
+void launch_hyb(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve, hipStream_t s) {
+  if (ve <= vb) return;
+  constexpr int U = G == 1 ? 4 : 2;
+  hipLaunchKernelGGL((k_ms_hyb_pull<K, G, U>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s,
+                     a, vb, ve);
+  G500_CHECK_LAUNCH();
+}
+
+template <int K, int G>
+void launch_hyb_push(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve,
+                     hipStream_t s) {
+  vb = std::max(vb, a.hot);
+  if (ve <= vb) return;
+  hipLaunchKernelGGL((k_ms_hyb_push<K, G>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s, a,
+                     vb, ve);
+  G500_CHECK_LAUNCH();
+}
+
 template <int K>
 void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
-  const MsLevel a = level_args(b, L);
+  MsLevel a = level_args(b, L);
+  a.hot = std::min<int64_t>(b->g->n_c, b->tuning.hot_bytes / (8 * K));
   const int64_t h = b->bin_heavy, m = b->bin_medium, n_c = b->g->n_c;
   const int chunk = b->tuning.heavy_chunk;
   const int64_t bw = bitmap_words(b);
@@ -991,6 +1181,35 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
         1, std::min<int64_t>(ntiles, (int64_t)num_cus(b->g->device) * b->tuning.blocks_per_cu));
     hipLaunchKernelGGL((k_ms_bu_stream<K>), dim3(sgrid), dim3(kT), 0, s, a, b->g->m);
     G500_CHECK_LAUNCH();
+    if (b->n_items > 0) {
+      hipLaunchKernelGGL((k_ms_hyb_pull_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
+                         b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
+      G500_CHECK_LAUNCH();
+    }
+    switch (b->tuning.medium_group) {
+      case 2: launch_hyb<K, 2>(b, a, h, m, s); break;
+      case 4: launch_hyb<K, 4>(b, a, h, m, s); break;
+      case 8: launch_hyb<K, 8>(b, a, h, m, s); break;
+      case 16: launch_hyb<K, 16>(b, a, h, m, s); break;
+      default: launch_hyb<K, 32>(b, a, h, m, s); break;
+    }
+    launch_hyb<K, 1>(b, a, m, n_c, s);
+    // heavy rows at or above the cut-off push by work item
+    const int64_t first_cold = a.hot < h ? (int64_t)b->h_item_start[(size_t)a.hot] : b->n_items;
+    if (first_cold < b->n_items) {
+      hipLaunchKernelGGL((k_ms_hyb_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
+                         b->item_start.get(), b->item_vertex.get(), first_cold, b->n_items,
+                         chunk);
+      G500_CHECK_LAUNCH();
+    }
+    switch (b->tuning.medium_group) {
+      case 2: launch_hyb_push<K, 2>(b, a, h, m, s); break;
+      case 4: launch_hyb_push<K, 4>(b, a, h, m, s); break;
+      case 8: launch_hyb_push<K, 8>(b, a, h, m, s); break;
+      case 16: launch_hyb_push<K, 16>(b, a, h, m, s); break;
+      default: launch_hyb_push<K, 32>(b, a, h, m, s); break;
+    }
+    launch_hyb_push<K, 1>(b, a, m, n_c, s);
     hipLaunchKernelGGL((k_ms_commit<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a);
     G500_CHECK_LAUNCH();
   }
@@ -1146,6 +1365,9 @@ void update_bins(g500_msbfs* b) {
                                   hipMemcpyDeviceToHost, s));
     G500_CHECK_HIP(hipStreamSynchronize(s));
     b->n_items = (int64_t)total;
+    b->h_item_start.resize((size_t)h + 1);
+    G500_CHECK_HIP(hipMemcpy(b->h_item_start.data(), b->item_start.get(),
+                             ((size_t)h + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     b->item_vertex.alloc((size_t)std::max<int64_t>(1, b->n_items));
     hipLaunchKernelGGL(k_ms_item_vertex, dim3((unsigned)ceil_div(h, kT)), dim3(kT), 0, s,
                        b->item_start.get(), h, b->item_vertex.get());
@@ -1210,7 +1432,9 @@ void msbfs_init(g500_msbfs* b, int max_roots) {
   t.use_graph = 1;
   t.heavy_chunk = 4096;
   t.td_alpha = 4.0;
-  t.stream_frac = 0.0;  // the streaming pull measured slower on Kronecker graphs
+  t.stream_frac = 0.0;  // dense levels off until measured faster (tests force them)
+  t.hot_bytes = 2 << 20;
+  t.dense_algo = kDenseHybrid;
   msbfs_set_tuning(b, t);
   G500_CHECK_HIP(hipStreamSynchronize(b->stream));
 }
@@ -1229,6 +1453,9 @@ void msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning& t) {
   if (!(t.td_alpha > 0)) fail(G500_ERR_ARG, "tuning: td_alpha must be > 0");
   if (!(t.stream_frac >= 0 && t.stream_frac <= 1))
     fail(G500_ERR_ARG, "tuning: stream_frac must be in [0, 1]");
+  if (t.dense_algo != kDenseStream && t.dense_algo != kDenseHybrid)
+    fail(G500_ERR_ARG, "tuning: dense_algo is 1 (stream) or 2 (hub pull + push)");
+  if (t.hot_bytes < 0) fail(G500_ERR_ARG, "tuning: hot_bytes must be >= 0");
   b->tuning = t;
   drop_graphs(b);
   update_bins(b);
@@ -1394,9 +1621,12 @@ double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int round
   const int32_t chunk[] = {1024, 2048, 4096, 8192, 16384};
   const double alpha[] = {1, 2, 4, 8, 16, 32, 64};
   const double sfrac[] = {0.0, 0.05, 0.2};
+  const int64_t hotb[] = {1 << 20, 2 << 20, 4 << 20, 8 << 20, 32 << 20};
+  const int32_t dalgo[] = {kDenseStream, kDenseHybrid};
   for (int round = 0; round < rounds; ++round) {
-    for (int p = 0; p < 7; ++p) {
-      const int nc = p == 0 ? 6 : p == 1 ? 6 : p == 2 ? 4 : p == 3 ? 4 : p == 4 ? 5 : p == 5 ? 7 : 3;
+    for (int p = 0; p < 9; ++p) {
+      static const int ncand[9] = {6, 6, 4, 4, 5, 7, 3, 5, 2};
+      const int nc = ncand[p];
       for (int c = 0; c < nc; ++c) {
         g500_msbfs_tuning t = best;
         switch (p) {
@@ -1406,7 +1636,9 @@ double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int round
           case 3: t.blocks_per_cu = bpc[c]; break;
           case 4: t.heavy_chunk = chunk[c]; break;
           case 5: t.td_alpha = alpha[c]; break;
-          default: t.stream_frac = sfrac[c]; break;
+          case 6: t.stream_frac = sfrac[c]; break;
+          case 7: t.hot_bytes = hotb[c]; break;
+          default: t.dense_algo = dalgo[c]; break;
         }
         if (t.medium_degree > t.heavy_degree) continue;
         if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;

@@ -44,6 +44,7 @@ struct g500_msbfs {
   unsigned long long td_threshold = 0;      // top-down while frontier edges < this
   g500::DevBuf<uint64_t> item_start;        // heavy work items: prefix by vertex (h+1)
   g500::DevBuf<uint32_t> item_vertex;       // owner of each heavy work item
+  std::vector<uint64_t> h_item_start;       // host copy of item_start
   int64_t n_items = 0;
   g500::DevBuf<uint64_t> acc;               // heavy pull accumulators (h * kmax)
   g500::ScanWorkspace scan_ws;

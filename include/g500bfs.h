@@ -229,10 +229,13 @@ typedef struct g500_msbfs g500_msbfs;
  * cut into work items of heavy_chunk edges, one wave per item; those with
  * degree >= medium_degree get medium_group lanes each; the rest one lane
  * each.  A level runs top-down (push from the frontier) while its frontier's
- * summed degree is below m / td_alpha, else bottom-up: while fewer than
- * stream_frac of all (root, vertex) pairs have been reached, as one balanced
- * stream over every edge (early exit rarely pays then), later per vertex with
- * early exit.
+ * summed degree is below m / td_alpha, else bottom-up.  While fewer than
+ * stream_frac of all (root, vertex) pairs have been reached, a bottom-up level
+ * is "dense" (early exit rarely pays) and runs as dense_algo says: 1 = one
+ * balanced stream over every edge; 2 = every vertex pulls from its neighbours
+ * below the hub cut-off (the lowest IDs, whose lane words take hot_bytes and
+ * stay cached), and the frontier vertices above it push along their rows.
+ * Later bottom-up levels pull per vertex with early exit.
  * blocks_per_cu sizes the grid of the persistent sweep kernels; use_graph = 1
  * replays the whole batch as one captured HIP graph. */
 typedef struct g500_msbfs_tuning {
@@ -243,7 +246,10 @@ typedef struct g500_msbfs_tuning {
   int32_t use_graph;     /* 0 or 1 */
   int32_t heavy_chunk;   /* 256..65536, multiple of 256 */
   double td_alpha;       /* > 0 */
-  double stream_frac;    /* 0..1; 0 = never stream */
+  double stream_frac;    /* 0..1; 0 = no dense levels */
+  int64_t hot_bytes;     /* dense_algo 2: lane words of the lowest IDs kept hot */
+  int32_t dense_algo;    /* dense levels: 1 streaming pull, 2 hub pull + push */
+  int32_t reserved;      /* 0 */
 } g500_msbfs_tuning;
 
 typedef struct g500_msbfs_stats {

@@ -6,7 +6,7 @@ the plain GPU engine on larger ones.  Traversed-edge counts must equal the
 validator's TE for the same root.  Covered: 1..256 roots (ragged lane words,
 duplicate roots, roots without edges), every degree-bin layout, graph replay
 against eager launches, every direction mode (top-down, streaming pull,
-per-vertex pull) forced on every level, the asynchronous launch/query/wait
+hub pull + push, per-vertex pull) forced on every level, the asynchronous launch/query/wait
 form, BFS depths beyond the captured level count, and the uint8 depth limit.
 """
 import numpy as np
@@ -109,8 +109,14 @@ def test_scale16_256_roots_equal_plain_engine(G, golden):
     dict(use_graph=0),
     dict(td_alpha=1e-9),                          # every level top-down
     dict(td_alpha=1e12, stream_frac=0.0),         # every level per-vertex pull
-    dict(td_alpha=1e12, stream_frac=1.0),         # every level streaming pull
-    dict(td_alpha=1e12, stream_frac=1.0, heavy_degree=1, medium_degree=1),
+    dict(td_alpha=1e12, stream_frac=1.0, dense_algo=1),   # every level streaming pull
+    dict(td_alpha=1e12, stream_frac=1.0, dense_algo=1, heavy_degree=1, medium_degree=1),
+    dict(td_alpha=1e12, stream_frac=1.0, dense_algo=2, hot_bytes=4096),  # hub pull + push
+    dict(td_alpha=1e12, stream_frac=1.0, dense_algo=2, hot_bytes=0),     # push only
+    dict(td_alpha=1e12, stream_frac=1.0, dense_algo=2, hot_bytes=1 << 40),  # pull only
+    dict(td_alpha=1e12, stream_frac=1.0, dense_algo=2, hot_bytes=512, heavy_degree=8,
+         medium_degree=4, heavy_chunk=256, medium_group=4),
+    dict(td_alpha=4.0, stream_frac=0.3, dense_algo=2, hot_bytes=2048),
     dict(heavy_degree=16, medium_degree=4, heavy_chunk=256),
     dict(heavy_degree=16, medium_degree=4, heavy_chunk=65536, td_alpha=1.0),
 ])
