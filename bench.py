@@ -123,15 +123,17 @@ def cpu_baseline(graph, roots, te_by_root, budget_s, args):
 
 
 def batched(g500bfs, graph, bfs, args, plain_teps):
-    """Batched bit-parallel BFS on the same graph: tuned on one root set,
-    timed on another, levels spot-checked against the plain engine."""
+    """Batched bit-parallel BFS on the same graph: tuned per batch size on one
+    root set, timed on another, levels spot-checked against the plain engine."""
     mb = g500bfs.MultiBfs(graph, max_roots=256)
     tune_roots = graph.find_roots(256, r1=101)
-    tuning, tune_ms = mb.autotune(tune_roots, rounds=1)
-    log(f"batched tuning {tuning} ({tune_ms:.2f} ms per 256-root batch)")
+    default = mb.tuning()
     roots = graph.find_roots(256, r1=0)
-    out = {"tuning": tuning}
+    out = {}
     for nr in (64, 128, 256):
+        mb.set_tuning(**default)
+        tuning, tune_ms = mb.autotune(tune_roots[:nr], rounds=1)
+        log(f"batched {nr}: tuning {tuning} ({tune_ms:.2f} ms per batch of the tuning roots)")
         r = roots[:nr]
         mb.run(r)
         host, dev = [], []
@@ -149,7 +151,8 @@ def batched(g500bfs, graph, bfs, args, plain_teps):
             exact &= bool(np.array_equal(mb.levels(i),
                                          np.where(l16 == 0xFFFF, 0xFF, l16).astype(np.uint8)))
         teps = float(te.sum()) / t
-        out[str(nr)] = {"ms_per_batch": round(t * 1e3, 3),
+        out[str(nr)] = {"tuning": tuning,
+                        "ms_per_batch": round(t * 1e3, 3),
                         "gpu_ms_per_batch": round(float(np.median(dev)), 3),
                         "roots_per_s": round(nr / t, 1),
                         "traversed_edges_per_s": round(teps, 1),
@@ -171,7 +174,7 @@ def serving(g500bfs, args):
         graphs.append(g)
         pools.append(g.find_roots(512))
     srv = serve.Server(graphs, max_batch=256)
-    srv.prewarm(1, pools)
+    srv.prewarm(4, pools)
     qs = serve.poisson_workload(graphs, pools, args.serve_queries, args.serve_qps, seed=7)
     t0 = time.perf_counter()
     done = serve.run_open_loop(srv, qs)

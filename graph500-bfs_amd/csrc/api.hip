@@ -399,13 +399,14 @@ int g500_bfs_destroy(g500_bfs* b) {
   });
 }
 
-int g500_msbfs_footprint(const g500_graph* g, int max_roots, int64_t* bytes) {
+int g500_msbfs_footprint(const g500_graph* g, int max_roots, int flags, int64_t* bytes) {
   return guard([&] {
     need_graph(g, true);
     if (!bytes) g500::fail(G500_ERR_ARG, "null output pointer");
     if (max_roots < 1 || max_roots > G500_MSBFS_MAX_ROOTS)
       g500::fail(G500_ERR_ARG, "max_roots %d outside [1, %d]", max_roots, G500_MSBFS_MAX_ROOTS);
-    *bytes = g500::msbfs_footprint(g, max_roots);
+    if (flags & ~G500_MSBFS_NO_LEVEL_ARRAYS) g500::fail(G500_ERR_ARG, "unknown flags 0x%x", flags);
+    *bytes = g500::msbfs_footprint(g, max_roots, !(flags & G500_MSBFS_NO_LEVEL_ARRAYS));
   });
 }
 
@@ -421,7 +422,12 @@ int g500_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
 }
 
 int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out) {
+  return g500_msbfs_create_ex(g, max_roots, 0, out);
+}
+
+int g500_msbfs_create_ex(g500_graph* g, int max_roots, int flags, g500_msbfs** out) {
   return guard([&] {
+    if (flags & ~G500_MSBFS_NO_LEVEL_ARRAYS) g500::fail(G500_ERR_ARG, "unknown flags 0x%x", flags);
     need_graph(g, true);
     if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
     *out = nullptr;
@@ -433,7 +439,7 @@ int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out) {
     auto* b = new g500_msbfs;
     try {
       b->g = g;
-      g500::msbfs_init(b, max_roots);
+      g500::msbfs_init(b, max_roots, !(flags & G500_MSBFS_NO_LEVEL_ARRAYS));
     } catch (...) {
       delete b;
       throw;
@@ -493,6 +499,8 @@ int g500_msbfs_get_levels(g500_msbfs* b, int index, uint8_t* host_levels) {
   return guard([&] {
     need_result(b);
     if (!host_levels) g500::fail(G500_ERR_ARG, "null output buffer");
+    if (!b->level_arrays)
+      g500::fail(G500_ERR_STATE, "engine created without level arrays (use point lookups)");
     if (index < 0 || index >= b->nroots)
       g500::fail(G500_ERR_ARG, "root index %d outside [0, %d)", index, b->nroots);
     const size_t n = (size_t)b->g->n;
@@ -505,6 +513,8 @@ int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride)
   return guard([&] {
     need_result(b);
     if (!dev_ptr || !row_stride) g500::fail(G500_ERR_ARG, "null output pointer");
+    if (!b->level_arrays)
+      g500::fail(G500_ERR_STATE, "engine created without level arrays (use point lookups)");
     *dev_ptr = b->out.get();
     *row_stride = b->g->n;
   });

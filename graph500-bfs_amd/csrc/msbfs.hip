@@ -994,14 +994,34 @@ __global__ __launch_bounds__(kT) void k_ms_output_tile(const int64_t* __restrict
   }
 }
 
-// levels[i] = out[root_index[i]][vertex[i]] (0xFF for vertices outside [0, n))
-__global__ void k_ms_gather(const uint8_t* __restrict__ out, int64_t n,
+// levels[i] = level of original vertex pairs[2i+1] for batch root pairs[2i],
+// read from the level frontiers: the l with bit r of F_l[v] (0xFF if none or
+// if the vertex is outside [0, n)); a root without edges is at level 0.
+__global__ void k_ms_gather(const int64_t* __restrict__ d_roots,
+                            const uint32_t* __restrict__ o2n, int64_t n, int64_t n_c, int K,
+                            const uint64_t* const* __restrict__ fptr,
+                            const unsigned long long* __restrict__ cnt, int cap,
                             const int64_t* __restrict__ pairs, int64_t count,
                             uint8_t* __restrict__ levels) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const int64_t r = pairs[2 * i], v = pairs[2 * i + 1];
-  levels[i] = (v >= 0 && v < n) ? out[r * n + v] : (uint8_t)0xFF;
+  const int64_t r = pairs[2 * i], o = pairs[2 * i + 1];
+  uint8_t lev = 0xFF;
+  if (o >= 0 && o < n) {
+    const int64_t v = o2n[o];
+    if (v < n_c) {
+      const int nlev = ms_levels(cnt, cap);
+      const uint64_t bit = 1ull << (r & 63);
+      for (int l = 0; l < nlev; ++l)
+        if (fptr[l][v * K + (r >> 6)] & bit) {
+          lev = (uint8_t)l;
+          break;
+        }
+    } else if (d_roots[1 + r] == o) {
+      lev = 0;
+    }
+  }
+  levels[i] = lev;
 }
 
 // Roots without edges reach only themselves.
@@ -1016,7 +1036,7 @@ __global__ void k_ms_isolated(const int64_t* __restrict__ d_roots,
   const int64_t o = d_roots[1 + i];
   const int64_t v = o2n[o];
   if (v < n_c) return;
-  out[(int64_t)i * n + o] = 0;
+  if (out) out[(int64_t)i * n + o] = 0;
   te[i] = cnt0[v];
 }
 
@@ -1245,7 +1265,9 @@ template <int K>
 void launch_output(const g500_msbfs* b, hipStream_t s) {
   const g500_graph* g = b->g;
   const int cap = b->levels_cap();
-  if (g->n % kT == 0) {
+  if (!b->level_arrays) {
+    // point lookups read the level frontiers; no per-root arrays
+  } else if (g->n % kT == 0) {
     const unsigned grid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>(g->n / kT, (int64_t)num_cus(g->device) * 8));
     hipLaunchKernelGGL((k_ms_output_tile<K>), dim3(grid), dim3(kT), 0, s, b->roots.get(),
@@ -1265,7 +1287,7 @@ void launch_output(const g500_msbfs* b, hipStream_t s) {
   G500_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ms_isolated, dim3(1), dim3(G500_MSBFS_MAX_ROOTS), 0, s, b->roots.get(),
                      g->orig2new.get(), g->n, g->n_c, b->cnt0.get(), b->cnt.get(), cap,
-                     b->out.get(), b->te.get());
+                     b->level_arrays ? b->out.get() : nullptr, b->te.get());
   G500_CHECK_LAUNCH();
 }
 
@@ -1379,20 +1401,20 @@ void update_bins(g500_msbfs* b) {
 
 }  // namespace
 
-int64_t msbfs_footprint(const g500_graph* g, int max_roots) {
+int64_t msbfs_footprint(const g500_graph* g, int max_roots, bool level_arrays) {
   const int64_t kmax = ceil_div(max_roots, 64);
   const int64_t nc = std::max<int64_t>(1, g->n_c);
   const int64_t bw = std::max<int64_t>(1, ceil_div(g->n_c, 64));
   int64_t bytes = nc * kmax * 8;                      // seen
   bytes += (int64_t)kInitialLevels * nc * kmax * 8;   // level frontiers
-  bytes += (int64_t)max_roots * g->n;                 // level arrays
+  if (level_arrays) bytes += (int64_t)max_roots * g->n;  // level arrays
   bytes += g->n * 4;                                  // cnt0
   bytes += 2 * bw * 8;                                // frontier bitmaps
   bytes += 4 * (kMaxLevels + 2) * 8 + 8 * 1024;       // counters, roots, te, tables
   return bytes;
 }
 
-void msbfs_init(g500_msbfs* b, int max_roots) {
+void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
   g500_graph* g = b->g;
   b->max_roots = max_roots;
   b->kmax = (int)ceil_div(max_roots, 64);
@@ -1405,7 +1427,8 @@ void msbfs_init(g500_msbfs* b, int max_roots) {
   b->cnt.alloc(kMaxLevels + 2);
   b->te.alloc(G500_MSBFS_MAX_ROOTS);
   b->roots.alloc(G500_MSBFS_MAX_ROOTS + 1);
-  b->out.alloc((size_t)max_roots * (size_t)g->n);
+  b->level_arrays = level_arrays;
+  if (level_arrays) b->out.alloc((size_t)max_roots * (size_t)g->n);
   b->fb[0].alloc((size_t)bitmap_words(b));
   b->fb[1].alloc((size_t)bitmap_words(b));
   b->fe.alloc(kMaxLevels + 2);
@@ -1583,17 +1606,21 @@ void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* l
     if (pairs[2 * i] < 0 || pairs[2 * i] >= b->nroots)
       fail(G500_ERR_ARG, "pair %lld: root index %lld outside [0, %d)", (long long)i,
            (long long)pairs[2 * i], b->nroots);
-  DevBuf<int64_t> dp;
-  DevBuf<uint8_t> dl;
-  dp.alloc((size_t)count * 2);
-  dl.alloc((size_t)count);
+  if ((int64_t)b->gather_pairs.n < 2 * count) {
+    b->gather_pairs.alloc((size_t)2 * count);
+    b->gather_levels.alloc((size_t)count);
+  }
+  const g500_graph* g = b->g;
   hipStream_t s = b->stream;
-  G500_CHECK_HIP(hipMemcpyAsync(dp.get(), pairs, (size_t)count * 2 * sizeof(int64_t),
+  G500_CHECK_HIP(hipMemcpyAsync(b->gather_pairs.get(), pairs, (size_t)count * 2 * sizeof(int64_t),
                                 hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_ms_gather, dim3((unsigned)ceil_div(count, kT)), dim3(kT), 0, s,
-                     b->out.get(), b->g->n, dp.get(), count, dl.get());
+                     b->roots.get(), g->orig2new.get(), g->n, g->n_c, b->K,
+                     (const uint64_t* const*)b->fptr.get(), b->cnt.get(), b->levels_cap(),
+                     b->gather_pairs.get(), count, b->gather_levels.get());
   G500_CHECK_LAUNCH();
-  G500_CHECK_HIP(hipMemcpyAsync(levels, dl.get(), (size_t)count, hipMemcpyDeviceToHost, s));
+  G500_CHECK_HIP(hipMemcpyAsync(levels, b->gather_levels.get(), (size_t)count,
+                                hipMemcpyDeviceToHost, s));
   G500_CHECK_HIP(hipStreamSynchronize(s));
 }
 

@@ -35,6 +35,9 @@ struct g500_msbfs {
   int64_t* h_roots = nullptr;               // pinned staging for the roots
   unsigned long long* h_cnt = nullptr;      // pinned copy of cnt after a run
   g500::DevBuf<uint8_t> out;                // levels [max_roots][n] by original ID
+  bool level_arrays = true;                 // false: point lookups only (no `out`)
+  g500::DevBuf<int64_t> gather_pairs;       // point-lookup staging
+  g500::DevBuf<uint8_t> gather_levels;
 
   g500::DevBuf<uint64_t> fb[2];             // frontier bitmaps (level & 1), n_c bits
   g500::DevBuf<unsigned long long> fe;      // frontier degree sum per level [cap + 2]
@@ -61,9 +64,9 @@ struct g500_msbfs {
 };
 
 namespace g500 {
-void msbfs_init(g500_msbfs* b, int max_roots);
+void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays);
 // device bytes msbfs_init allocates (before the heavy work-item tables)
-int64_t msbfs_footprint(const g500_graph* g, int max_roots);
+int64_t msbfs_footprint(const g500_graph* g, int max_roots, bool level_arrays);
 void msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning& t);
 void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots);
 bool msbfs_query(g500_msbfs* b);

@@ -95,7 +95,8 @@ def load():
         "g500_bfs_get_stats": [vp, C.POINTER(RunStats), C.POINTER(LevelTrace), C.c_int],
         "g500_bfs_destroy": [vp],
         "g500_msbfs_create": [vp, C.c_int, C.POINTER(vp)],
-        "g500_msbfs_footprint": [vp, C.c_int, i64p],
+        "g500_msbfs_footprint": [vp, C.c_int, C.c_int, i64p],
+        "g500_msbfs_create_ex": [vp, C.c_int, C.c_int, C.POINTER(vp)],
         "g500_device_memory": [C.c_int, i64p, i64p],
         "g500_msbfs_set_tuning": [vp, C.POINTER(MsbfsTuning)],
         "g500_msbfs_get_tuning": [vp, C.POINTER(MsbfsTuning)],
@@ -129,9 +130,13 @@ def device_memory(device=0):
     return f.value, t.value
 
 
-def msbfs_footprint(graph, max_roots):
+MSBFS_NO_LEVEL_ARRAYS = 1
+
+
+def msbfs_footprint(graph, max_roots, level_arrays=True):
     b = C.c_int64()
-    _check(load().g500_msbfs_footprint(graph.h, int(max_roots), C.byref(b)))
+    flags = 0 if level_arrays else MSBFS_NO_LEVEL_ARRAYS
+    _check(load().g500_msbfs_footprint(graph.h, int(max_roots), flags, C.byref(b)))
     return b.value
 
 
@@ -310,10 +315,14 @@ class Bfs:
 class MultiBfs:
     """Batched bit-parallel BFS: up to 256 roots per launch, uint8 levels."""
 
-    def __init__(self, graph, max_roots=MSBFS_MAX_ROOTS):
+    def __init__(self, graph, max_roots=MSBFS_MAX_ROOTS, level_arrays=True):
         self.graph = graph
         h = C.c_void_p()
-        _check(load().g500_msbfs_create(graph.h, int(max_roots), C.byref(h)))
+        flags = 0 if level_arrays else MSBFS_NO_LEVEL_ARRAYS
+        if flags:
+            _check(load().g500_msbfs_create_ex(graph.h, int(max_roots), flags, C.byref(h)))
+        else:
+            _check(load().g500_msbfs_create(graph.h, int(max_roots), C.byref(h)))
         self.h = h
         self.nroots = 0
 

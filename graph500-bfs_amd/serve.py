@@ -64,9 +64,11 @@ class Server:
         self.budget = int(hbm_budget)
         self.used = 0
         self.max_engines = int(max_engines_per_graph)
-        self.new_engine = engine_factory or (lambda g, r: G.MultiBfs(g, max_roots=r))
+        # answers need point lookups only, so engines keep no level arrays
+        self.new_engine = engine_factory or \
+            (lambda g, r: G.MultiBfs(g, max_roots=r, level_arrays=False))
         self.footprint = list(footprint) if footprint is not None else \
-            [G.msbfs_footprint(g, self.max_batch) for g in self.graphs]
+            [G.msbfs_footprint(g, self.max_batch, level_arrays=False) for g in self.graphs]
         self.engines = [[] for _ in self.graphs]
         self.queues = [collections.deque() for _ in self.graphs]
         self.done = []
@@ -227,7 +229,7 @@ def one_per_launch(graphs, queries, clock=time.perf_counter):
     """Baseline: every query alone, one launch each, back to back (plain
     single-root engine for the BFS, batch-of-one engine for the answer)."""
     bfs = [G.Bfs(g) for g in graphs]
-    one = [G.MultiBfs(g, max_roots=1) for g in graphs]
+    one = [G.MultiBfs(g, max_roots=1, level_arrays=False) for g in graphs]
     service = []
     plain = []
     for q in queries:

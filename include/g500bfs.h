@@ -260,13 +260,21 @@ typedef struct g500_msbfs_stats {
   int64_t lane_discoveries[64]; /* per level (first 64 levels) */
 } g500_msbfs_stats;
 
-/* Device bytes g500_msbfs_create(g, max_roots) will allocate (admission
- * control of a serving loop checks it against g500_device_memory). */
-int g500_msbfs_footprint(const g500_graph* g, int max_roots, int64_t* bytes);
+/* flags of g500_msbfs_create_ex / g500_msbfs_footprint */
+enum {
+  G500_MSBFS_NO_LEVEL_ARRAYS = 1  /* answer point lookups only: no [roots][N] arrays */
+};
+/* Device bytes g500_msbfs_create_ex(g, max_roots, flags) will allocate
+ * (admission control of a serving loop checks it against g500_device_memory). */
+int g500_msbfs_footprint(const g500_graph* g, int max_roots, int flags, int64_t* bytes);
 /* Free and total device memory (hipMemGetInfo). */
 int g500_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
 /* Allocates the engine for batches of up to max_roots roots (1..256). */
 int g500_msbfs_create(g500_graph* g, int max_roots, g500_msbfs** out);
+/* The same with flags (G500_MSBFS_NO_LEVEL_ARRAYS: no per-root level arrays;
+ * g500_msbfs_get_levels and g500_msbfs_device_levels then fail, point lookups
+ * work). */
+int g500_msbfs_create_ex(g500_graph* g, int max_roots, int flags, g500_msbfs** out);
 int g500_msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning* t);
 int g500_msbfs_get_tuning(g500_msbfs* b, g500_msbfs_tuning* t);
 /* Enqueues a batch (roots by original ID, 1..max_roots of them, duplicates
@@ -285,7 +293,8 @@ int g500_msbfs_get_levels(g500_msbfs* b, int index, uint8_t* host_levels);
 int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride);
 /* Point lookups into the last batch: levels[i] = level of vertex pairs[2i+1]
  * (original ID) for the root at batch index pairs[2i] (0xFF = unreached or
- * vertex out of range). */
+ * vertex out of range); read from the level frontiers, so they work with or
+ * without level arrays. */
 int g500_msbfs_gather_levels(g500_msbfs* b, const int64_t* pairs, int64_t count,
                              uint8_t* levels);
 /* Picks the tuning fields by timing batches of `roots` (coordinate search,

@@ -247,3 +247,28 @@ def test_autotune_keeps_levels(G, s12):
     mb.run(roots)
     for i in (0, 2, 3, 40, 63):
         assert np.array_equal(mb.levels(i), _u8(og.bfs(int(roots[i]))))
+
+
+def test_engine_without_level_arrays(G, s12):
+    print("\n[msbfs] no level arrays", flush=True)
+    g, og = s12
+    roots = _mixed_roots(g, og, 90, seed=17)
+    assert G.msbfs_footprint(g, 90, level_arrays=False) + 90 * g.info.n == \
+        G.msbfs_footprint(g, 90)
+    mb = G.MultiBfs(g, max_roots=90, level_arrays=False)
+    mb.run(roots)
+    ref = G.MultiBfs(g, max_roots=90)
+    ref.run(roots)
+    assert np.array_equal(mb.te(), ref.te())
+    rng = np.random.default_rng(2)
+    ri = rng.integers(0, 90, 400)
+    vs = rng.integers(0, g.info.n, 400)
+    vs[:90] = roots  # every root looked up at itself (level 0, also without edges)
+    ri[:90] = np.arange(90)
+    assert np.array_equal(mb.gather(ri, vs), ref.gather(ri, vs))
+    assert np.all(mb.gather(np.arange(90), roots) == 0)
+    for i in (0, 2, 50):
+        exp = _u8(og.bfs(int(roots[i])))
+        assert np.array_equal(mb.gather(np.full(g.info.n, i), np.arange(g.info.n)), exp)
+    with pytest.raises(G.G500Error):
+        mb.levels(0)

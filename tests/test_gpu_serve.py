@@ -57,7 +57,7 @@ def test_mixed_stream_answers_equal_alone(setup, max_batch, engines):
 def test_budget_for_one_engine_per_graph(setup):
     print("\n[serve] budget = one engine per graph", flush=True)
     G, serve, graphs, oracles, pools = setup
-    need = sum(G.msbfs_footprint(g, 64) for g in graphs)
+    need = sum(G.msbfs_footprint(g, 64, level_arrays=False) for g in graphs)
     srv = serve.Server(graphs, max_batch=64, hbm_budget=need)
     qs = serve.poisson_workload(graphs, pools, 300, rate_qps=1e5, seed=9)
     done = serve.run_open_loop(srv, qs)
