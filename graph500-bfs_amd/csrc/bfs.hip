@@ -3,17 +3,19 @@
 // Level loop and alpha/beta switch follow BfsBase::run_bfs_core
 // (mpi/bfs.hpp:3238-3566, decision at 3413-3457).  The data layout is our
 // own: vertices renumbered by descending degree ("new" IDs), a uint64 visited
-// bitmap over new IDs, uint32 frontier queues, and pred written directly by
-// original ID as packed (parent | depth << 48) entries.
+// bitmap over new IDs, uint32 frontier queues, and packed (parent | depth <<
+// 48) entries written by new ID during the levels and gathered into pred by
+// original ID at the end.
 //
 // Kernels
-//   k_td_expand   top-down: one prefix sum over frontier degrees, then a
-//                 persistent grid walks 2048-edge tiles; the tile's slice of
-//                 the prefix sum sits in LDS and each lane binary-searches
-//                 its edge's owner there (every lane gets one edge, whatever
-//                 the degree skew).  Winners of the visited-bit atomicOr write
-//                 pred and are appended to an LDS queue that is flushed with
-//                 one global atomic per tile.
+//   k_td_expand   top-down: one prefix sum over frontier degrees and a
+//                 tile -> owner table (k_td_tiles), then a persistent grid
+//                 walks 2048-edge tiles; the tile's slice of the prefix sum
+//                 sits in LDS and each lane binary-searches its edges' owners
+//                 there (every lane gets eight edges, whatever the degree
+//                 skew).  Winners of the visited-bit atomicOr write pred and
+//                 are appended to an LDS queue that is flushed with one global
+//                 atomic per tile.
 //   k_bu_step     bottom-up: a wave reads 64 consecutive bitmap words
 //                 (coalesced), compacts the unvisited vertices onto its 64
 //                 lanes with shuffles, and each lane scans its vertex's row
@@ -160,9 +162,25 @@ __device__ inline int64_t owner_search(const uint64_t* a, int64_t lo,
   return lo;
 }
 
+// tile_owner[t] = frontier index whose edge range holds edge t * kTdTile
+// (each tile start lies in exactly one range: frontier rows are non-empty);
+// the last vertex also writes the entry after the last tile.
+__global__ __launch_bounds__(kT) void k_td_tiles(const uint64_t* __restrict__ dsum,
+                                                int64_t qlen,
+                                                uint32_t* __restrict__ tile_owner) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= qlen) return;
+  const uint64_t a = dsum[i], b = dsum[i + 1];
+  if (b <= a) return;
+  const uint64_t t0 = (a + kTdTile - 1) / kTdTile, t1 = (b - 1) / kTdTile;
+  for (uint64_t t = t0; t <= t1; ++t) tile_owner[t] = (uint32_t)i;
+  if (i == qlen - 1) tile_owner[t1 + 1] = (uint32_t)i;
+}
+
 __global__ __launch_bounds__(kT) void k_td_expand(
     const uint32_t* __restrict__ q, int64_t qlen,
-    const uint64_t* __restrict__ dsum, const int64_t* __restrict__ row_off,
+    const uint64_t* __restrict__ dsum, const uint32_t* __restrict__ tile_owner,
+    const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint32_t* __restrict__ n2o,
     const uint32_t* __restrict__ deg_new, unsigned long long* vis,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ nq, LevelSync* ls,
@@ -181,9 +199,11 @@ __global__ __launch_bounds__(kT) void k_td_expand(
        e0 += (uint64_t)gridDim.x * kTdTile) {
     const uint64_t e1 = (e0 + kTdTile < E) ? e0 + kTdTile : E;
     if (t == 0) {
-      const int64_t lo = owner_search(dsum, 0, qlen - 1, e0);
-      s_lo = lo;
-      s_hi = owner_search(dsum, lo, qlen - 1, e1 - 1);
+      // owners of the tile's first edge and of the next tile's first edge
+      // (>= the owner of its last edge)
+      const uint64_t tile = e0 / kTdTile;
+      s_lo = tile_owner[tile];
+      s_hi = min((int64_t)tile_owner[tile + 1], qlen - 1);
       lq_n = 0;
     }
     __syncthreads();
@@ -476,6 +496,8 @@ void bfs_init(g500_bfs* b) {
   b->q.alloc((size_t)std::max<int64_t>(g->n_c, 1));
   b->nq.alloc((size_t)std::max<int64_t>(g->n_c, 1));
   b->dsum.alloc((size_t)std::max<int64_t>(g->n_c, 1) + 1);
+  // a frontier's edges never exceed m
+  b->tile_owner.alloc((size_t)ceil_div(std::max<int64_t>(g->m, 1), kTdTile) + 2);
   b->lsync.alloc(1);
   G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
   b->scan_ws.reserve(std::max<int64_t>(g->n_c, 1));
@@ -632,9 +654,14 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
         G500_CHECK_LAUNCH();
       }
       exclusive_scan<uint64_t>(b->dsum.get(), b->dsum.get(), qlen, b->scan_ws, s);
+      if (qlen > 0) {
+        hipLaunchKernelGGL(k_td_tiles, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT), 0, s,
+                           b->dsum.get(), qlen, b->tile_owner.get());
+        G500_CHECK_LAUNCH();
+      }
       const unsigned grid = est_edges < 0 ? 256u : grid_for(est_edges, kTdTile);
       hipLaunchKernelGGL(k_td_expand, dim3(grid), dim3(kT), 0, s,
-                         b->q.get(), qlen, b->dsum.get(), g->row_off.get(),
+                         b->q.get(), qlen, b->dsum.get(), b->tile_owner.get(), g->row_off.get(),
                          g->col.get(), g->new2orig.get(), g->deg_new.get(),
                          (unsigned long long*)b->vis.get(), b->pred_new.get(), b->nq.get(),
                          ls, b->d_report, seq, level_bits);

@@ -29,6 +29,7 @@ struct g500_bfs {
   g500::DevBuf<uint64_t> vis, vis_alt;  // visited bitmaps (new IDs)
   g500::DevBuf<uint32_t> q, nq;     // frontier queues (new IDs)
   g500::DevBuf<uint64_t> dsum;      // frontier degree prefix sums
+  g500::DevBuf<uint32_t> tile_owner; // frontier index owning each top-down tile's first edge
   g500::DevBuf<LevelSync> lsync;   // per-level counters
   HostReport* h_report = nullptr;  // pinned host memory, written by the GPU
   HostReport* d_report = nullptr;  // its device address

@@ -134,6 +134,12 @@ __global__ void k_ms_roots(const int64_t* __restrict__ d_roots,
   atomicAdd(&cnt[0], 1ull);
 }
 
+__device__ inline uint32_t ld_col(const uint32_t* p) { return *p; }
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline u32x4_t ld_col4(const uint32_t* p) {
+  return *reinterpret_cast<const u32x4_t*>(p);
+}
+
 template <int K>
 __device__ inline void load_lanes(const uint64_t* __restrict__ p, uint64_t (&x)[K]) {
   if constexpr (K == 2 || K == 4) {
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(kT) void k_ms_td(MsLevel a, int64_t vb, int64_t ve)
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
     const int64_t end = a.row_off[v + 1];
-    for (int64_t e = a.row_off[v] + lig; e < end; e += G) ms_push<K>(a, a.col[e], f, disc, edges);
+    for (int64_t e = a.row_off[v] + lig; e < end; e += G) ms_push<K>(a, ld_col(a.col + e), f, disc, edges);
   }
   ms_flush(a, red, disc, edges);
 }
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
     load_lanes<K>(a.fl + v * K, f);
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e = beg + lane; e < end; e += kWave) ms_push<K>(a, a.col[e], f, disc, edges);
+    for (int64_t e = beg + lane; e < end; e += kWave) ms_push<K>(a, ld_col(a.col + e), f, disc, edges);
   }
   ms_flush(a, red, disc, edges);
 }
@@ -298,6 +304,10 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
 // LDS accumulators of the tile's vertex window; the window then goes to
 // F_{L+1} (stored for rows inside the tile, atomicOr for the rows that cross
 // its ends).  k_ms_commit masks the result with seen afterwards.
+// Tile bounds of the streaming pull come from a per-graph table: entry t is
+// the vertex whose row holds edge t * kStreamUnit.
+constexpr int kStreamUnit = 1024;
+
 template <int K>
 struct StreamTile {
   static constexpr int kTile = K <= 2 ? 2048 : 1024;
@@ -315,11 +325,13 @@ __device__ inline void stream_flush(unsigned long long* acc_l, int64_t li, uint6
 }
 
 template <int K>
-__global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m) {
+__global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m,
+                                                    const uint32_t* __restrict__ tile_v0) {
   constexpr int kTile = StreamTile<K>::kTile, kEpt = StreamTile<K>::kEpt;
-  __shared__ int64_t win[kTile + 1];
-  __shared__ unsigned long long acc_l[kTile * K];
-  __shared__ uint8_t full_l[kTile];
+  constexpr int kStep = kTile / kStreamUnit;  // table entries per tile
+  __shared__ int64_t win[kTile + 2];
+  __shared__ unsigned long long acc_l[(kTile + 1) * K];
+  __shared__ uint8_t full_l[kTile + 1];
   __shared__ int64_t s_v0, s_wn;
   if (!ms_active(a) || a.dense_algo != kDenseStream || ms_mode(a, a.L) != kModeStream) return;
   const int t = threadIdx.x;
@@ -327,22 +339,13 @@ __global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m) {
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t e0 = tile * kTile, e1 = min(m, e0 + kTile);
     if (t == 0) {
-      // rows are non-empty, so the window holds at most kTile vertices
-      int64_t lo = 0, hi = a.n_c - 1;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (a.row_off[mid] <= e0) lo = mid;
-        else hi = mid - 1;
-      }
-      const int64_t v0 = lo;
-      hi = min(a.n_c - 1, v0 + kTile - 1);
-      while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (a.row_off[mid] <= e1 - 1) lo = mid;
-        else hi = mid - 1;
-      }
+      // owner of the tile's first edge, and of the next tile's first edge
+      // (>= the owner of its last); rows are non-empty, so the window holds
+      // at most kTile + 1 vertices
+      const int64_t v0 = tile_v0[tile * kStep];
+      const int64_t v1 = min((int64_t)tile_v0[(tile + 1) * kStep], a.n_c - 1);
       s_v0 = v0;
-      s_wn = lo - v0 + 1;
+      s_wn = v1 - v0 + 1;
     }
     __syncthreads();
     const int64_t v0 = s_v0, wn = s_wn;
@@ -363,10 +366,9 @@ __global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m) {
       // phase 1: targets (vector loads) and owners (one search, then advance)
       uint32_t u[kEpt];
       if (eb + kEpt <= e1) {
-        const uint4* p = reinterpret_cast<const uint4*>(a.col + eb);
 #pragma unroll
         for (int j = 0; j < kEpt / 4; ++j) {
-          const uint4 y = p[j];
+          const u32x4_t y = ld_col4(a.col + eb + 4 * j);
           u[4 * j] = y.x;
           u[4 * j + 1] = y.y;
           u[4 * j + 2] = y.z;
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m) {
         }
       } else {
 #pragma unroll
-        for (int j = 0; j < kEpt; ++j) u[j] = eb + j < e1 ? a.col[eb + j] : 0u;
+        for (int j = 0; j < kEpt; ++j) u[j] = eb + j < e1 ? ld_col(a.col + eb + j) : 0u;
       }
       int64_t lo = 0, hi = wn - 1;
       while (lo < hi) {
@@ -471,7 +473,7 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_pull(MsLevel a, int64_t vb, int64
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         const int64_t e = base + (int64_t)j * G + lig;
-        u[j] = e < end ? a.col[e] : 0xFFFFFFFFu;
+        u[j] = e < end ? ld_col(a.col + e) : 0xFFFFFFFFu;
         past = past || (e < end && u[j] >= hot);
       }
 #pragma unroll
@@ -531,7 +533,7 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_pull_heavy(MsLevel a,
     for (int k = 0; k < K; ++k) acc[k] = 0;
     for (int64_t base = beg; base < end; base += kWave) {
       const int64_t e = base + lane;
-      const uint32_t u = e < end ? a.col[e] : 0xFFFFFFFFu;
+      const uint32_t u = e < end ? ld_col(a.col + e) : 0xFFFFFFFFu;
       if (u < hot && in_bitmap(a.fb, u)) {
         uint64_t x[K];
         load_lanes<K>(a.fl + (int64_t)u * K, x);
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push(MsLevel a, int64_t vb, int64
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
     const int64_t end = a.row_off[v + 1];
-    for (int64_t e = a.row_off[v] + lig; e < end; e += G) hyb_push_one<K>(a, a.col[e], f);
+    for (int64_t e = a.row_off[v] + lig; e < end; e += G) hyb_push_one<K>(a, ld_col(a.col + e), f);
   }
 }
 
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push_heavy(MsLevel a,
     load_lanes<K>(a.fl + v * K, f);
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e = beg + lane; e < end; e += kWave) hyb_push_one<K>(a, a.col[e], f);
+    for (int64_t e = beg + lane; e < end; e += kWave) hyb_push_one<K>(a, ld_col(a.col + e), f);
   }
 }
 
@@ -678,7 +680,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
 #pragma unroll
           for (int j = 0; j < U; ++j) {
             const int64_t e = base + (int64_t)j * G + lig;
-            u[j] = e < end ? a.col[e] : 0xFFFFFFFFu;
+            u[j] = e < end ? ld_col(a.col + e) : 0xFFFFFFFFu;
           }
 #pragma unroll
           for (int j = 0; j < U; ++j) {
@@ -772,7 +774,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy(MsLevel a,
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         const int64_t e = base + (int64_t)j * kWave + lane;
-        u[j] = e < end ? a.col[e] : 0xFFFFFFFFu;
+        u[j] = e < end ? ld_col(a.col + e) : 0xFFFFFFFFu;
       }
 #pragma unroll
       for (int j = 0; j < U; ++j) {
@@ -1199,7 +1201,8 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
     const int64_t ntiles = ceil_div(b->g->m, StreamTile<K>::kTile);
     const unsigned sgrid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>(ntiles, (int64_t)num_cus(b->g->device) * b->tuning.blocks_per_cu));
-    hipLaunchKernelGGL((k_ms_bu_stream<K>), dim3(sgrid), dim3(kT), 0, s, a, b->g->m);
+    hipLaunchKernelGGL((k_ms_bu_stream<K>), dim3(sgrid), dim3(kT), 0, s, a, b->g->m,
+                       b->stream_tile_v0.get());
     G500_CHECK_LAUNCH();
     if (b->n_items > 0) {
       hipLaunchKernelGGL((k_ms_hyb_pull_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
@@ -1343,6 +1346,18 @@ void drop_graphs(g500_msbfs* b) {
   }
 }
 
+__global__ void k_ms_stream_tiles(const int64_t* __restrict__ row_off, int64_t n_c,
+                                  uint32_t* __restrict__ tile_v0) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_c) return;
+  const int64_t a = row_off[v], b = row_off[v + 1];
+  if (b <= a) return;
+  const int64_t t0 = (a + kStreamUnit - 1) / kStreamUnit, t1 = (b - 1) / kStreamUnit;
+  for (int64_t t = t0; t <= t1; ++t) tile_v0[t] = (uint32_t)v;
+  if (v == n_c - 1)  // entries past the last edge point at the last vertex
+    for (int64_t t = t1 + 1; t <= t1 + 2; ++t) tile_v0[t] = (uint32_t)v;
+}
+
 __global__ void k_ms_item_vertex(const uint64_t* __restrict__ item_start, int64_t h,
                                  uint32_t* __restrict__ item_vertex) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1411,6 +1426,7 @@ int64_t msbfs_footprint(const g500_graph* g, int max_roots, bool level_arrays) {
   bytes += g->n * 4;                                  // cnt0
   bytes += 2 * bw * 8;                                // frontier bitmaps
   bytes += 4 * (kMaxLevels + 2) * 8 + 8 * 1024;       // counters, roots, te, tables
+  bytes += (ceil_div(g->m, kStreamUnit) + 3) * 4;     // streaming tile table
   return bytes;
 }
 
@@ -1445,6 +1461,13 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
         1, std::min<int64_t>(ceil_div(g->nedges, kT), (int64_t)num_cus(g->device) * 16));
     hipLaunchKernelGGL(k_ms_cnt0, dim3(grid), dim3(kT), 0, b->stream, g->edges.get(),
                        g->nedges, g->orig2new.get(), b->cnt0.get());
+    G500_CHECK_LAUNCH();
+  }
+  if (g->m > 0) {
+    // tiles of 2048 edges read entries 2t and 2t+2: room for both past the end
+    b->stream_tile_v0.alloc((size_t)ceil_div(g->m, kStreamUnit) + 3);
+    hipLaunchKernelGGL(k_ms_stream_tiles, dim3((unsigned)ceil_div(g->n_c, kT)), dim3(kT), 0,
+                       b->stream, g->row_off.get(), g->n_c, b->stream_tile_v0.get());
     G500_CHECK_LAUNCH();
   }
   g500_msbfs_tuning t{};

@@ -36,6 +36,7 @@ struct g500_msbfs {
   unsigned long long* h_cnt = nullptr;      // pinned copy of cnt after a run
   g500::DevBuf<uint8_t> out;                // levels [max_roots][n] by original ID
   bool level_arrays = true;                 // false: point lookups only (no `out`)
+  g500::DevBuf<uint32_t> stream_tile_v0;    // streaming pull: first vertex per 1024 edges
   g500::DevBuf<int64_t> gather_pairs;       // point-lookup staging
   g500::DevBuf<uint8_t> gather_levels;
 

@@ -6,9 +6,14 @@ run; the --pmc passes give FETCH_SIZE and WRITE_SIZE per kernel per run, each
 scaled by what the same counter reads for a device-to-device copy of known
 size (pass the calibration runs' CSV files).
 
+For the batched engine a run is the span from one k_ms_init to the next;
+`batched` groups consecutive runs (e.g. "1,5" = skip one, summarise five) and
+reports the median time per level and kernel.
+
 usage:
   python rocprof_summary.py times  KT_KERNEL_TRACE.csv
   python rocprof_summary.py traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_BYTES
+  python rocprof_summary.py batched KT_KERNEL_TRACE.csv GROUPS
 """
 import collections
 import csv
@@ -88,8 +93,41 @@ def traffic(fetch, write, cfetch, cwrite, copy_bytes):
         print(f"  {k:26s} fetch {statistics.median(fk.get(k, [0]))/1e6:9.1f} MB  write {statistics.median(wk.get(k, [0]))/1e6:9.1f} MB")
 
 
+def batched(path, groups):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows = [r for r in rows if "k_ms_" in r["Kernel_Name"]]
+    idx = [i for i, r in enumerate(rows) if "k_ms_init" in r["Kernel_Name"]] + [len(rows)]
+    runs_ = [rows[a:b] for a, b in zip(idx, idx[1:])]
+    pos = 0
+    spec = [int(x) for x in groups.split(",")]
+    for gi in range(0, len(spec), 2):
+        skip, take = spec[gi], spec[gi + 1]
+        sel = runs_[pos + skip:pos + skip + take]
+        pos += skip + take
+        spans = [(int(r[-1]["End_Timestamp"]) - int(r[0]["Start_Timestamp"])) / 1e3 for r in sel]
+        per = collections.defaultdict(list)
+        for run in sel:
+            level, c = -1, collections.Counter()
+            for r in run:
+                k = short(r["Kernel_Name"])
+                if k.startswith("k_ms_prep"):
+                    level += 1
+                tag = f"L{level}" if level >= 0 and not k.startswith(("k_ms_output", "k_ms_te", "k_ms_isolated")) else "out"
+                c[(tag, k)] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            for key in set(per) | set(c):
+                per[key].append(c.get(key, 0.0))
+        print(f"runs {pos - take}..{pos - 1}: batch span median {statistics.median(spans):.1f} us")
+        for (tag, k), v in sorted(per.items(), key=lambda kv: (kv[0][0] != "out", kv[0][0])):
+            m = statistics.median(v)
+            if m >= 20:
+                print(f"  {tag:4s} {k:28s} {m:9.1f} us")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "times":
         times(sys.argv[2])
+    elif sys.argv[1] == "batched":
+        batched(sys.argv[2], sys.argv[3])
     else:
         traffic(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], int(sys.argv[6]))
