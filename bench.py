@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched multi-source BFS measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--batched-no-tune", action="store_true",
+                    help="batched leg with the default tuning (no autotuning; for profiling)")
     ap.add_argument("--serve-queries", type=int, default=4000)
     ap.add_argument("--serve-qps", type=float, default=2000.0)
     return ap.parse_args()
@@ -132,8 +134,11 @@ def batched(g500bfs, graph, bfs, args, plain_teps):
     out = {}
     for nr in (64, 128, 256):
         mb.set_tuning(**default)
-        tuning, tune_ms = mb.autotune(tune_roots[:nr], rounds=1)
-        log(f"batched {nr}: tuning {tuning} ({tune_ms:.2f} ms per batch of the tuning roots)")
+        if args.batched_no_tune:
+            tuning = default
+        else:
+            tuning, tune_ms = mb.autotune(tune_roots[:nr], rounds=1)
+            log(f"batched {nr}: tuning {tuning} ({tune_ms:.2f} ms per batch of the tuning roots)")
         r = roots[:nr]
         mb.run(r)
         host, dev = [], []

@@ -14,6 +14,7 @@ usage:
   python rocprof_summary.py times  KT_KERNEL_TRACE.csv
   python rocprof_summary.py traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_BYTES
   python rocprof_summary.py batched KT_KERNEL_TRACE.csv GROUPS
+  python rocprof_summary.py batched-traffic FETCH.csv WRITE.csv FETCH_SCALE WRITE_SCALE GROUPS
 """
 import collections
 import csv
@@ -124,10 +125,31 @@ def batched(path, groups):
                 print(f"  {tag:4s} {k:28s} {m:9.1f} us")
 
 
+def batched_traffic(fetch, write, fscale, wscale, groups):
+    res = {}
+    for label, path, cname, f in (("fetch", fetch, "FETCH_SIZE", fscale),
+                                  ("write", write, "WRITE_SIZE", wscale)):
+        c = [x for x in _counter(path, cname) if "k_ms_" in x[1]]
+        idx = [i for i, x in enumerate(c) if "k_ms_init" in x[1]] + [len(c)]
+        res[label] = [sum(x[2] * 1024.0 * f for x in c[a:b]) for a, b in zip(idx, idx[1:])]
+    spec = [int(x) for x in groups.split(",")]
+    pos = 0
+    for gi in range(0, len(spec), 2):
+        skip, take = spec[gi], spec[gi + 1]
+        fs = res["fetch"][pos + skip:pos + skip + take]
+        ws = res["write"][pos + skip:pos + skip + take]
+        pos += skip + take
+        print(f"runs {pos - take}..{pos - 1}: fetch {statistics.median(fs)/1e9:.2f} GB, "
+              f"write {statistics.median(ws)/1e9:.2f} GB per batch (median)")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "times":
         times(sys.argv[2])
     elif sys.argv[1] == "batched":
         batched(sys.argv[2], sys.argv[3])
+    elif sys.argv[1] == "batched-traffic":
+        batched_traffic(sys.argv[2], sys.argv[3], float(sys.argv[4]), float(sys.argv[5]),
+                        sys.argv[6])
     else:
         traffic(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], int(sys.argv[6]))
