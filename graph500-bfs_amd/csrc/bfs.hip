@@ -50,6 +50,7 @@ constexpr int kT = 256;
 constexpr int kWavesPerBlock = kT / kWave;
 constexpr int kTdItems = 8;
 constexpr int kTdTile = kT * kTdItems;
+constexpr int kTdWin = 512;  // owners staged in LDS per tile (more: read from memory)
 constexpr int kMaxGrid = 2048;
 constexpr int kBuIlp = 4;  // vertices in flight per lane in k_bu_step
 
@@ -151,6 +152,11 @@ __global__ __launch_bounds__(kT) void k_td_degrees(const uint32_t* __restrict__ 
   }
 }
 
+// Filter before the visited-bit atomicOr (which alone decides the winner).
+__device__ inline unsigned long long probe_vis(unsigned long long* p) {
+  return __atomic_load_n(p, __ATOMIC_RELAXED);
+}
+
 // largest i in [lo, hi] with a[i] <= e   (a non-decreasing, a[lo] <= e)
 __device__ inline int64_t owner_search(const uint64_t* a, int64_t lo,
                                        int64_t hi, uint64_t e) {
@@ -162,9 +168,10 @@ __device__ inline int64_t owner_search(const uint64_t* a, int64_t lo,
   return lo;
 }
 
-// tile_owner[t] = frontier index whose edge range holds edge t * kTdTile
+// tile_owner[t] = frontier index whose edge range holds edge t * kTile
 // (each tile start lies in exactly one range: frontier rows are non-empty);
 // the last vertex also writes the entry after the last tile.
+template <int kTile>
 __global__ __launch_bounds__(kT) void k_td_tiles(const uint64_t* __restrict__ dsum,
                                                 int64_t qlen,
                                                 uint32_t* __restrict__ tile_owner) {
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(kT) void k_td_tiles(const uint64_t* __restrict__ ds
   if (i >= qlen) return;
   const uint64_t a = dsum[i], b = dsum[i + 1];
   if (b <= a) return;
-  const uint64_t t0 = (a + kTdTile - 1) / kTdTile, t1 = (b - 1) / kTdTile;
+  const uint64_t t0 = (a + kTile - 1) / kTile, t1 = (b - 1) / kTile;
   for (uint64_t t = t0; t <= t1; ++t) tile_owner[t] = (uint32_t)i;
   if (i == qlen - 1) tile_owner[t1 + 1] = (uint32_t)i;
 }
@@ -185,7 +192,14 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     const uint32_t* __restrict__ deg_new, unsigned long long* vis,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ nq, LevelSync* ls,
     HostReport* hr, unsigned long long seq, uint64_t level_bits) {
-  __shared__ uint64_t win[kTdTile];
+  // the tile's owners, staged once per tile: prefix sum, col offset of edge
+  // 0 of the owner's range (row_off[u] - dsum[i]) and original ID.  On heavy
+  // levels a tile has one or two owners (hubs) shared by every block at
+  // once; reading them per lane from memory would pile thousands of
+  // requests onto a few cache lines
+  __shared__ uint64_t win[kTdWin];
+  __shared__ int64_t wrb[kTdWin];
+  __shared__ uint32_t wpu[kTdWin];
   __shared__ uint32_t lq[kTdTile];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ int64_t s_lo, s_hi;
@@ -209,9 +223,15 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     __syncthreads();
     const int64_t lo = s_lo, hi = s_hi;
     const int64_t wn = hi - lo + 1;
-    const bool use_win = wn <= kTdTile;
+    const bool use_win = wn <= kTdWin;
     if (use_win)
-      for (int64_t j = t; j < wn; j += kT) win[j] = dsum[lo + j];
+      for (int64_t j = t; j < wn; j += kT) {
+        const uint64_t d = dsum[lo + j];
+        const uint32_t u = q[lo + j];
+        win[j] = d;
+        wrb[j] = row_off[u] - (int64_t)d;
+        wpu[j] = n2o[u];
+      }
     __syncthreads();
     // item j of lane t is edge e0 + j*kT + t (coalesced col reads); the loads
     // of all items are issued phase by phase so their latencies overlap
@@ -223,29 +243,28 @@ __global__ __launch_bounds__(kT) void k_td_expand(
       for (int j = 0; j < kTdItems; ++j) {
         const uint64_t e = e0 + (uint64_t)j * kT + t;
         act[j] = e < e1;
-        int64_t i = lo;
-        uint64_t base = 0;
+        kk[j] = 0;
+        pu[j] = 0;
         if (act[j]) {
           if (use_win) {
             const int64_t li = owner_search(win, 0, wn - 1, e);
-            i = lo + li;
-            base = win[li];
+            kk[j] = wrb[li] + (int64_t)e;
+            pu[j] = wpu[li];
           } else {
-            i = owner_search(dsum, lo, hi, e);
-            base = dsum[i];
+            const int64_t i = owner_search(dsum, lo, hi, e);
+            const uint32_t u = q[i];
+            kk[j] = row_off[u] + (int64_t)(e - dsum[i]);
+            pu[j] = n2o[u];
           }
         }
-        const uint32_t u = act[j] ? q[i] : 0u;
-        kk[j] = act[j] ? row_off[u] + (int64_t)(e - base) : 0;
-        pu[j] = act[j] ? n2o[u] : 0u;
       }
 #pragma unroll
       for (int j = 0; j < kTdItems; ++j) v[j] = act[j] ? col[kk[j]] : 0u;
     }
-    unsigned long long w[kTdItems];
+    uint64_t w[kTdItems];
 #pragma unroll
     for (int j = 0; j < kTdItems; ++j)
-      w[j] = act[j] ? __atomic_load_n(vis + (v[j] >> 6), __ATOMIC_RELAXED) : ~0ull;
+      w[j] = act[j] ? probe_vis(vis + (v[j] >> 6)) : ~0ull;
     uint32_t nwon = 0;
     bool won[kTdItems];
 #pragma unroll
@@ -655,8 +674,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       }
       exclusive_scan<uint64_t>(b->dsum.get(), b->dsum.get(), qlen, b->scan_ws, s);
       if (qlen > 0) {
-        hipLaunchKernelGGL(k_td_tiles, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT), 0, s,
-                           b->dsum.get(), qlen, b->tile_owner.get());
+        hipLaunchKernelGGL(k_td_tiles<kTdTile>, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT),
+                           0, s, b->dsum.get(), qlen, b->tile_owner.get());
         G500_CHECK_LAUNCH();
       }
       const unsigned grid = est_edges < 0 ? 256u : grid_for(est_edges, kTdTile);
