@@ -16,11 +16,14 @@
 //                 skew).  Winners of the visited-bit atomicOr write pred and
 //                 are appended to an LDS queue that is flushed with one global
 //                 atomic per tile.
-//   k_bu_step     bottom-up: a wave reads 64 consecutive bitmap words
-//                 (coalesced), compacts the unvisited vertices onto its 64
-//                 lanes with shuffles, and each lane scans its vertex's row
-//                 until the first visited neighbour (early exit).  New bits
-//                 gather in LDS and are written once per word.
+//   k_bu_sweep    first bottom-up level after top-down ones: a wave owns 64
+//                 bitmap words; lane l takes bit l of each word, so the
+//                 first-neighbour probes read `first` coalesced, eight
+//                 vertices per lane in flight.  The still-unvisited vertices
+//                 go to per-chunk lists.
+//   k_bu_list     later bottom-up levels: the same probes over those lists,
+//                 compacted in place as vertices are found.  New bits are
+//                 written once per bitmap word either way.
 //   k_bitmap_to_queue   bitmap -> queue at the bottom-up -> top-down switch.
 //   k_pred_gather after the last level: pred[o] for every original ID o,
 //                 gathered from the entries the levels wrote by new ID
@@ -52,7 +55,6 @@ constexpr int kTdItems = 8;
 constexpr int kTdTile = kT * kTdItems;
 constexpr int kTdWin = 512;  // owners staged in LDS per tile (more: read from memory)
 constexpr int kMaxGrid = 2048;
-constexpr int kBuIlp = 4;  // vertices in flight per lane in k_bu_step
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -312,109 +314,211 @@ __global__ __launch_bounds__(kT) void k_td_expand(
   }
 }
 
-__global__ __launch_bounds__(kT) void k_bu_step(
+// Bottom-up levels.  A wave owns a chunk of 64 bitmap words (4096 vertices)
+// and, for it, a segment of 4096 entries in the unvisited lists: after a
+// bottom-up level the segment holds the chunk's still-unvisited vertices in
+// ascending order and unv_cnt[chunk] their count.  The first bottom-up level
+// after top-down levels sweeps the bitmap (k_bu_sweep); later ones walk the
+// lists (k_bu_list), which shrink level by level.  Either way a vertex first
+// probes its highest-degree neighbour (the dense `first` copy: nearly every
+// vertex found in a bottom-up level is found there) and only the rest scan
+// the remainder of their row; every probe of a batch is issued before any of
+// them is used.  A visited neighbour of a vertex unvisited at the start of a
+// level lies on the previous level, so the visited bitmap serves as the
+// frontier (bottom_up_search_bitmap, mpi/bfs.hpp:2828-2978; the reference
+// also keeps unvisited vertices as lists, swap_visited_memory and
+// bottom_up_search_list, mpi/bfs.hpp:1747-1779 and 2979-3005).  The parent
+// is the first visited neighbour in row order.
+constexpr int kBuIlp = 8;   // vertices in flight per lane
+constexpr int kChunkVerts = kWave * 64;
+// The visited bits of the lowest new IDs (the highest degrees) sit in LDS:
+// most first neighbours are such hubs, so most probes never leave the CU.
+constexpr int kHubWords = 4096;
+
+// Visited bit of u: from the block's LDS copy below hub_verts, else memory.
+struct VisProbe {
+  const uint64_t* hub;
+  const uint64_t* vis;
+  uint32_t hub_verts;
+  __device__ bool operator()(uint32_t u) const {
+    const uint64_t w = u < hub_verts ? hub[u >> 6] : vis[u >> 6];
+    return (w >> (u & 63)) & 1ull;
+  }
+};
+
+// Copies the first min(words, kHubWords) words of vis into LDS (all threads).
+__device__ inline VisProbe stage_hubs(uint64_t* hub, const uint64_t* vis, int64_t words) {
+  const int hw = (int)(words < kHubWords ? words : kHubWords);
+  for (int i = threadIdx.x; i < hw; i += kT) hub[i] = vis[i];
+  __syncthreads();
+  return VisProbe{hub, vis, (uint32_t)hw * 64u};
+}
+
+__device__ inline uint64_t readlane64(uint64_t x, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Probes the batch: first neighbours, then the rest of the rows of the
+// vertices whose first neighbour is unvisited.  On return hit[j] tells
+// whether v[j] has a visited neighbour and par[j] names the first one.
+__device__ inline void bu_probe(const uint32_t (&v)[kBuIlp], const bool (&ok)[kBuIlp],
+                                bool (&hit)[kBuIlp], uint32_t (&par)[kBuIlp],
+                                const VisProbe& visited,
+                                const int64_t* __restrict__ row_off,
+                                const uint32_t* __restrict__ col,
+                                const uint32_t* __restrict__ first) {
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j) par[j] = ok[j] ? first[v[j]] : 0u;
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(par[j]);
+  int64_t k[kBuIlp], k1[kBuIlp];
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j) {
+    const bool scan = ok[j] && !hit[j];
+    k[j] = scan ? row_off[v[j]] + 1 : 0;
+    k1[j] = scan ? row_off[v[j] + 1] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j) {
+    while (k[j] < k1[j]) {
+      uint32_t u[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) u[t] = (k[j] + t < k1[j]) ? col[k[j] + t] : 0u;
+      bool h[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) h[t] = (k[j] + t < k1[j]) && visited(u[t]);
+#pragma unroll
+      for (int t = 3; t >= 0; --t)
+        if (h[t]) {
+          hit[j] = true;
+          par[j] = u[t];
+        }
+      k[j] = hit[j] ? k1[j] : k[j] + 4;
+    }
+  }
+}
+
+// pred entries of the batch's hits; returns the wave-uniform number of
+// misses appended to the chunk's unvisited segment at position wp.
+__device__ inline uint32_t bu_commit(const uint32_t (&v)[kBuIlp], const bool (&ok)[kBuIlp],
+                                     const bool (&hit)[kBuIlp], const uint32_t (&par)[kBuIlp],
+                                     const uint32_t* __restrict__ n2o,
+                                     int64_t* __restrict__ pred_new, uint64_t level_bits,
+                                     uint32_t* __restrict__ useg, uint32_t wp) {
+  uint32_t pp[kBuIlp];
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j) pp[j] = hit[j] ? n2o[par[j]] : 0u;
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j)
+    if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)pp[j] | level_bits);
+  const uint64_t below = lanemask_lt();
+  uint32_t added = 0;
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j) {
+    const bool miss = ok[j] && !hit[j];
+    const uint64_t mm = __ballot(miss);
+    if (miss) useg[wp + added + (uint32_t)__popcll(mm & below)] = v[j];
+    added += (uint32_t)__popcll(mm);
+  }
+  return added;
+}
+
+__global__ __launch_bounds__(kT) void k_bu_sweep(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint32_t* __restrict__ first,
     const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
-    LevelSync* ls, HostReport* hr, unsigned long long seq,
-    uint64_t level_bits) {
-  __shared__ unsigned long long fw[kWavesPerBlock][kWave];
+    uint32_t* __restrict__ unv, uint32_t* __restrict__ unv_cnt, LevelSync* ls,
+    HostReport* hr, unsigned long long seq, uint64_t level_bits) {
   __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ uint64_t hub[kHubWords];
+  const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t nchunks = ceil_div(words, kWave);
   uint64_t nfound = 0;
   for (int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + wave; c < nchunks;
        c += (int64_t)gridDim.x * kWavesPerBlock) {
     const int64_t w = c * kWave + lane;
-    const uint64_t word = w < words ? vis_old[w] : ~0ull;
-    const uint64_t um = ~word;
-    const uint32_t cnt = (uint32_t)__popcll(um);
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o, kWave);
-      if (lane >= o) incl += y;
-    }
-    const uint32_t excl = incl - cnt;
-    const uint32_t total = __shfl(incl, kWave - 1, kWave);
-    fw[wave][lane] = 0;
-    wave_sync();
-    // kBuIlp unvisited vertices per lane per round, loads issued together
-    for (uint32_t b0 = 0; b0 < total; b0 += kWave * kBuIlp) {
+    const uint64_t myword = w < words ? vis_old[w] : ~0ull;
+    uint32_t* useg = unv + c * kChunkVerts;
+    uint64_t newbits = 0;  // lane l: bits found in word l of the chunk
+    uint32_t wp = 0;
+    // lane l takes bit l of words j0 .. j0+kBuIlp-1: coalesced `first` reads
+    for (int j0 = 0; j0 < kWave; j0 += kBuIlp) {
       uint32_t v[kBuIlp], par[kBuIlp];
-      int wl[kBuIlp], bp[kBuIlp];
       bool ok[kBuIlp], hit[kBuIlp];
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
-        const uint32_t gidx = b0 + j * kWave + lane;
-        // owning word: largest l with excl_l <= gidx
-        int lo = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-          const int cand = lo + step;
-          const uint32_t e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
-          if (cand < kWave && e <= gidx) lo = cand;
-        }
-        const uint64_t wum = __shfl(um, lo, kWave);
-        const uint32_t elo = __shfl(excl, lo, kWave);
-        ok[j] = gidx < total;
-        uint32_t r = gidx - elo;
-        uint64_t x = wum;
-        int pos = 0;
-#pragma unroll
-        for (int half = 32; half > 0; half >>= 1) {
-          const uint32_t lc = (uint32_t)__popcll(x & ((1ull << half) - 1));
-          if (r >= lc) {
-            r -= lc;
-            x >>= half;
-            pos += half;
-          }
-        }
-        wl[j] = lo;
-        bp[j] = pos;
-        v[j] = (uint32_t)((c * kWave + lo) * 64 + pos);
+        const uint64_t wd = readlane64(myword, j0 + j);
+        ok[j] = !((wd >> lane) & 1ull);
+        v[j] = (uint32_t)((c * kWave + j0 + j) * 64 + lane);
       }
-      // first neighbour (highest degree) from the dense copy
+      bu_probe(v, ok, hit, par, visited, row_off, col, first);
 #pragma unroll
-      for (int j = 0; j < kBuIlp; ++j) par[j] = ok[j] ? first[v[j]] : 0u;
+      for (int j = 0; j < kBuIlp; ++j) {
+        const uint64_t hm = __ballot(hit[j]);
+        if (lane == j0 + j) newbits = hm;
+      }
+      wp += bu_commit(v, ok, hit, par, n2o, pred_new, level_bits, useg, wp);
+    }
+    if (w < words) vis_new[w] = myword | newbits;
+    nfound += (uint64_t)__popcll(newbits);
+    if (lane == 0) unv_cnt[c] = wp;
+  }
+  const uint64_t s = block_sum(nfound, red);
+  if (threadIdx.x == 0) {
+    if (s) atomicAdd(&ls->cnt[0], (unsigned long long)s);
+    finish_level(ls, hr, seq);
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_bu_list(
+    const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
+    int64_t words, const int64_t* __restrict__ row_off,
+    const uint32_t* __restrict__ col, const uint32_t* __restrict__ first,
+    const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
+    uint32_t* __restrict__ unv, uint32_t* __restrict__ unv_cnt, LevelSync* ls,
+    HostReport* hr, unsigned long long seq, uint64_t level_bits) {
+  __shared__ unsigned long long fw[kWavesPerBlock][kWave];
+  __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ uint64_t hub[kHubWords];
+  const VisProbe visited = stage_hubs(hub, vis_old, words);
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int64_t nchunks = ceil_div(words, kWave);
+  uint64_t nfound = 0;
+  for (int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + wave; c < nchunks;
+       c += (int64_t)gridDim.x * kWavesPerBlock) {
+    const int64_t w = c * kWave + lane;
+    const uint64_t myword = w < words ? vis_old[w] : ~0ull;
+    const uint32_t n = unv_cnt[c];
+    uint32_t* useg = unv + c * kChunkVerts;
+    const uint32_t vbase = (uint32_t)(c * kChunkVerts);
+    fw[wave][lane] = 0;
+    wave_sync();
+    // compaction in place: the write position never passes the batch read
+    uint32_t wp = 0;
+    for (uint32_t e0 = 0; e0 < n; e0 += kWave * kBuIlp) {
+      uint32_t v[kBuIlp], par[kBuIlp];
+      bool ok[kBuIlp], hit[kBuIlp];
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) {
+        const uint32_t i = e0 + (uint32_t)(j * kWave + lane);
+        ok[j] = i < n;
+        v[j] = ok[j] ? useg[i] : vbase;
+      }
+      bu_probe(v, ok, hit, par, visited, row_off, col, first);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j)
-        hit[j] = ok[j] && ((vis_old[par[j] >> 6] >> (par[j] & 63)) & 1ull);
-      // rest of the row, 4 independent loads at a time
-#pragma unroll
-      for (int j = 0; j < kBuIlp; ++j) {
-        if (!ok[j] || hit[j]) continue;
-        int64_t k = row_off[v[j]] + 1;
-        const int64_t k1 = row_off[v[j] + 1];
-        while (!hit[j] && k < k1) {
-          uint32_t u[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t) u[t] = (k + t < k1) ? col[k + t] : par[j];
-          bool h[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t)
-            h[t] = (k + t < k1) && ((vis_old[u[t] >> 6] >> (u[t] & 63)) & 1ull);
-#pragma unroll
-          for (int t = 3; t >= 0; --t)
-            if (h[t]) {
-              hit[j] = true;
-              par[j] = u[t];
-            }
-          k += 4;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kBuIlp; ++j) {
-        if (!hit[j]) continue;
-        atomicOr(&fw[wave][wl[j]], 1ull << bp[j]);
-        pred_new[v[j]] = (int64_t)((uint64_t)n2o[par[j]] | level_bits);
-      }
+        if (hit[j]) atomicOr(&fw[wave][(v[j] - vbase) >> 6], 1ull << (v[j] & 63));
+      wp += bu_commit(v, ok, hit, par, n2o, pred_new, level_bits, useg, wp);
     }
     wave_sync();
     const uint64_t fbits = fw[wave][lane];
-    if (w < words) vis_new[w] = word | fbits;
+    if (w < words) vis_new[w] = myword | fbits;
     nfound += (uint64_t)__popcll(fbits);
+    if (lane == 0) unv_cnt[c] = wp;
     wave_sync();
   }
   const uint64_t s = block_sum(nfound, red);
@@ -517,6 +621,21 @@ void bfs_init(g500_bfs* b) {
   b->dsum.alloc((size_t)std::max<int64_t>(g->n_c, 1) + 1);
   // a frontier's edges never exceed m
   b->tile_owner.alloc((size_t)ceil_div(std::max<int64_t>(g->m, 1), kTdTile) + 2);
+  const int64_t chunks = ceil_div(b->words, kWave);
+  b->unv.alloc((size_t)(chunks * kChunkVerts));
+  b->unv_cnt.alloc((size_t)chunks);
+  // bottom-up grids: what stays resident (each block stages the hub bits
+  // once), never more blocks than chunk groups
+  int cus = 0, per_cu[2] = {0, 0};
+  G500_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+  G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu[0], reinterpret_cast<const void*>(&k_bu_sweep), kT, 0));
+  G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu[1], reinterpret_cast<const void*>(&k_bu_list), kT, 0));
+  for (int i = 0; i < 2; ++i)
+    b->bu_grid[i] = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)std::max(cus, 1) * std::max(per_cu[i], 1),
+                             ceil_div(chunks, kWavesPerBlock)));
   b->lsync.alloc(1);
   G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
   b->scan_ws.reserve(std::max<int64_t>(g->n_c, 1));
@@ -563,6 +682,24 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
     }
   }
   for (int i = 0; i < 4; ++i) out[i] = __atomic_load_n(&hr->v[i], __ATOMIC_ACQUIRE);
+}
+
+// One bottom-up level: vis -> vis_alt, then the two swap.  `lists` walks
+// the unvisited lists the previous bottom-up level left, else the bitmap.
+void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_bits) {
+  g500_graph* g = b->g;
+  if (lists)
+    hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kT), 0, b->stream, b->vis.get(), b->vis_alt.get(),
+                       b->words, g->row_off.get(), g->col.get(), g->first.get(),
+                       g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
+                       b->lsync.get(), b->d_report, seq, level_bits);
+  else
+    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kT), 0, b->stream, b->vis.get(), b->vis_alt.get(),
+                       b->words, g->row_off.get(), g->col.get(), g->first.get(),
+                       g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
+                       b->lsync.get(), b->d_report, seq, level_bits);
+  G500_CHECK_LAUNCH();
+  std::swap(b->vis.p, b->vis_alt.p);
 }
 
 }  // namespace
@@ -630,6 +767,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   const int64_t beta_thr = (int64_t)((double)n_c / (beta * edgefactor * 2.0));
   LevelSync* ls = b->lsync.get();
   unsigned long long rep[4];
+  bool unv_ok = false;  // unvisited lists match the visited bitmap
 
   for (;;) {
     at[G500_AT_LEVEL] = ++level;
@@ -654,13 +792,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
         !td || (b->exec_alpha > 0 && est_edges > 0 &&
                 (double)est_edges > (double)g->m / b->exec_alpha);
     if (td && exec_bu) {
-      hipLaunchKernelGGL(k_bu_step, dim3(grid_for(ceil_div(words, kWave), kWavesPerBlock)),
-                         dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(), words,
-                         g->row_off.get(), g->col.get(), g->first.get(),
-                         g->new2orig.get(), b->pred_new.get(), ls, b->d_report, seq,
-                         level_bits);
-      G500_CHECK_LAUNCH();
-      std::swap(b->vis.p, b->vis_alt.p);
+      launch_bu(b, false, seq, level_bits);
+      unv_ok = true;
       seq = ++b->seq;
       hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)ceil_div(words, kT)), dim3(kT),
                          0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
@@ -685,14 +818,10 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                          (unsigned long long*)b->vis.get(), b->pred_new.get(), b->nq.get(),
                          ls, b->d_report, seq, level_bits);
       G500_CHECK_LAUNCH();
+      unv_ok = false;
     } else {
-      hipLaunchKernelGGL(k_bu_step, dim3(grid_for(ceil_div(words, kWave), kWavesPerBlock)),
-                         dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(), words,
-                         g->row_off.get(), g->col.get(), g->first.get(),
-                         g->new2orig.get(), b->pred_new.get(), ls, b->d_report, seq,
-                         level_bits);
-      G500_CHECK_LAUNCH();
-      std::swap(b->vis.p, b->vis_alt.p);
+      launch_bu(b, unv_ok, seq, level_bits);
+      unv_ok = true;
     }
     if (b->timing) G500_CHECK_HIP(hipEventRecord(lev1, s));
     wait_report(b, seq, rep);

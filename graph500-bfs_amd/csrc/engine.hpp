@@ -30,6 +30,10 @@ struct g500_bfs {
   g500::DevBuf<uint32_t> q, nq;     // frontier queues (new IDs)
   g500::DevBuf<uint64_t> dsum;      // frontier degree prefix sums
   g500::DevBuf<uint32_t> tile_owner; // frontier index owning each top-down tile's first edge
+  // unvisited vertices after a bottom-up level: 4096 entries per chunk of 64
+  // bitmap words, unv_cnt[chunk] of them in use
+  g500::DevBuf<uint32_t> unv, unv_cnt;
+  unsigned bu_grid[2] = {1, 1};     // resident blocks of the sweep / list kernels
   g500::DevBuf<LevelSync> lsync;   // per-level counters
   HostReport* h_report = nullptr;  // pinned host memory, written by the GPU
   HostReport* d_report = nullptr;  // its device address
