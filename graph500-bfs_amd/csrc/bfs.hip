@@ -316,20 +316,22 @@ __global__ __launch_bounds__(kT) void k_td_expand(
 
 // Bottom-up levels.  A wave owns a chunk of 64 bitmap words (4096 vertices)
 // and, for it, a segment of 4096 entries in the unvisited lists: after a
-// bottom-up level the segment holds the chunk's still-unvisited vertices in
-// ascending order and unv_cnt[chunk] their count.  The first bottom-up level
-// after top-down levels sweeps the bitmap (k_bu_sweep); later ones walk the
-// lists (k_bu_list), which shrink level by level.  Either way a vertex first
-// probes its highest-degree neighbour (the dense `first` copy: nearly every
-// vertex found in a bottom-up level is found there) and only the rest scan
-// the remainder of their row; every probe of a batch is issued before any of
-// them is used.  A visited neighbour of a vertex unvisited at the start of a
-// level lies on the previous level, so the visited bitmap serves as the
-// frontier (bottom_up_search_bitmap, mpi/bfs.hpp:2828-2978; the reference
-// also keeps unvisited vertices as lists, swap_visited_memory and
-// bottom_up_search_list, mpi/bfs.hpp:1747-1779 and 2979-3005).  The parent
-// is the first visited neighbour in row order.
+// bottom-up level the segment's first unv_cnt[chunk] entries are the chunk's
+// still-unvisited vertices.  The first bottom-up level after top-down ones
+// sweeps the bitmap (k_bu_sweep); later ones walk the lists (k_bu_list),
+// which shrink level by level.  A vertex first probes its highest-degree
+// neighbour (the dense `first` pairs also give that neighbour's original ID
+// for pred): nearly every vertex found in a bottom-up level is found there.
+// Only vertices whose row has more entries (the `multi` bitmap) scan the
+// rest of it; the sweep defers them to the back of the chunk's segment and
+// scans them together at the end of the chunk.  A visited neighbour of a
+// vertex unvisited at the start of a level lies on the previous level, so
+// the visited bitmap serves as the frontier (bottom_up_search_bitmap,
+// mpi/bfs.hpp:2828-2978; the reference also keeps unvisited vertices as
+// lists, swap_visited_memory and bottom_up_search_list, mpi/bfs.hpp:1747-1779
+// and 2979-3005).  The parent is the first visited neighbour in row order.
 constexpr int kBuIlp = 8;   // vertices in flight per lane
+constexpr int kBuBatch = kWave * kBuIlp;
 constexpr int kChunkVerts = kWave * 64;
 // The visited bits of the lowest new IDs (the highest degrees) sit in LDS:
 // most first neighbours are such hubs, so most probes never leave the CU.
@@ -360,28 +362,40 @@ __device__ inline uint64_t readlane64(uint64_t x, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Probes the batch: first neighbours, then the rest of the rows of the
-// vertices whose first neighbour is unvisited.  On return hit[j] tells
-// whether v[j] has a visited neighbour and par[j] names the first one.
-__device__ inline void bu_probe(const uint32_t (&v)[kBuIlp], const bool (&ok)[kBuIlp],
-                                bool (&hit)[kBuIlp], uint32_t (&par)[kBuIlp],
-                                const VisProbe& visited,
-                                const int64_t* __restrict__ row_off,
-                                const uint32_t* __restrict__ col,
-                                const uint32_t* __restrict__ first) {
+// First-neighbour probes of a batch; hits get their pred entries.
+__device__ inline void bu_first(const uint32_t (&v)[kBuIlp], const bool (&ok)[kBuIlp],
+                                bool (&hit)[kBuIlp], const VisProbe& visited,
+                                const uint2* __restrict__ first,
+                                int64_t* __restrict__ pred_new, uint64_t level_bits) {
+  uint2 f[kBuIlp];
 #pragma unroll
-  for (int j = 0; j < kBuIlp; ++j) par[j] = ok[j] ? first[v[j]] : 0u;
+  for (int j = 0; j < kBuIlp; ++j) f[j] = ok[j] ? first[v[j]] : make_uint2(0u, 0u);
 #pragma unroll
-  for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(par[j]);
+  for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(f[j].x);
+#pragma unroll
+  for (int j = 0; j < kBuIlp; ++j)
+    if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)f[j].y | level_bits);
+}
+
+// Scans the rows of scan[j] vertices past their first entry; hits get their
+// pred entries (first visited neighbour in row order).
+__device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[kBuIlp],
+                               bool (&hit)[kBuIlp], const VisProbe& visited,
+                               const int64_t* __restrict__ row_off,
+                               const uint32_t* __restrict__ col,
+                               const uint32_t* __restrict__ n2o,
+                               int64_t* __restrict__ pred_new, uint64_t level_bits) {
   int64_t k[kBuIlp], k1[kBuIlp];
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j) {
-    const bool scan = ok[j] && !hit[j];
-    k[j] = scan ? row_off[v[j]] + 1 : 0;
-    k1[j] = scan ? row_off[v[j] + 1] : 0;
+    k[j] = scan[j] ? row_off[v[j]] + 1 : 0;
+    k1[j] = scan[j] ? row_off[v[j] + 1] : 0;
   }
+  uint32_t par[kBuIlp];
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j) {
+    hit[j] = false;
+    par[j] = 0;
     while (k[j] < k1[j]) {
       uint32_t u[4];
 #pragma unroll
@@ -398,40 +412,43 @@ __device__ inline void bu_probe(const uint32_t (&v)[kBuIlp], const bool (&ok)[kB
       k[j] = hit[j] ? k1[j] : k[j] + 4;
     }
   }
-}
-
-// pred entries of the batch's hits; returns the wave-uniform number of
-// misses appended to the chunk's unvisited segment at position wp.
-__device__ inline uint32_t bu_commit(const uint32_t (&v)[kBuIlp], const bool (&ok)[kBuIlp],
-                                     const bool (&hit)[kBuIlp], const uint32_t (&par)[kBuIlp],
-                                     const uint32_t* __restrict__ n2o,
-                                     int64_t* __restrict__ pred_new, uint64_t level_bits,
-                                     uint32_t* __restrict__ useg, uint32_t wp) {
   uint32_t pp[kBuIlp];
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j) pp[j] = hit[j] ? n2o[par[j]] : 0u;
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j)
     if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)pp[j] | level_bits);
+}
+
+// Appends the lanes' flagged vertices at useg[pos ...] (ascending) or, with
+// down, at useg[pos - 1], useg[pos - 2], ...; returns how many.
+__device__ inline uint32_t bu_append(const uint32_t (&v)[kBuIlp], const bool (&flag)[kBuIlp],
+                                     uint32_t* __restrict__ useg, uint32_t pos, bool down) {
   const uint64_t below = lanemask_lt();
   uint32_t added = 0;
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j) {
-    const bool miss = ok[j] && !hit[j];
-    const uint64_t mm = __ballot(miss);
-    if (miss) useg[wp + added + (uint32_t)__popcll(mm & below)] = v[j];
-    added += (uint32_t)__popcll(mm);
+    const uint64_t m = __ballot(flag[j]);
+    const uint32_t r = added + (uint32_t)__popcll(m & below);
+    if (flag[j]) useg[down ? pos - 1 - r : pos + r] = v[j];
+    added += (uint32_t)__popcll(m);
   }
   return added;
+}
+
+__device__ inline bool multi_bit(const uint64_t* multi, uint32_t v) {
+  return (multi[v >> 6] >> (v & 63)) & 1ull;
 }
 
 __global__ __launch_bounds__(kT) void k_bu_sweep(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
-    const uint32_t* __restrict__ col, const uint32_t* __restrict__ first,
-    const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
-    uint32_t* __restrict__ unv, uint32_t* __restrict__ unv_cnt, LevelSync* ls,
-    HostReport* hr, unsigned long long seq, uint64_t level_bits) {
+    const uint32_t* __restrict__ col, const uint2* __restrict__ first,
+    const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
+    int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
+    uint32_t* __restrict__ unv_cnt, LevelSync* ls, HostReport* hr,
+    unsigned long long seq, uint64_t level_bits) {
+  __shared__ unsigned long long fw[kWavesPerBlock][kWave];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
@@ -442,30 +459,62 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
        c += (int64_t)gridDim.x * kWavesPerBlock) {
     const int64_t w = c * kWave + lane;
     const uint64_t myword = w < words ? vis_old[w] : ~0ull;
+    const uint64_t mymulti = w < words ? multi[w] : 0ull;
     uint32_t* useg = unv + c * kChunkVerts;
+    const uint32_t vbase = (uint32_t)(c * kChunkVerts);
     uint64_t newbits = 0;  // lane l: bits found in word l of the chunk
-    uint32_t wp = 0;
+    // final misses fill the segment from the front, deferred row scans from
+    // the back; together they never exceed the chunk's unvisited vertices
+    uint32_t wp = 0, dn = 0;
     // lane l takes bit l of words j0 .. j0+kBuIlp-1: coalesced `first` reads
     for (int j0 = 0; j0 < kWave; j0 += kBuIlp) {
-      uint32_t v[kBuIlp], par[kBuIlp];
-      bool ok[kBuIlp], hit[kBuIlp];
+      uint32_t v[kBuIlp];
+      bool ok[kBuIlp], hit[kBuIlp], fin[kBuIlp], defer[kBuIlp];
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         const uint64_t wd = readlane64(myword, j0 + j);
         ok[j] = !((wd >> lane) & 1ull);
-        v[j] = (uint32_t)((c * kWave + j0 + j) * 64 + lane);
+        v[j] = vbase + (uint32_t)((j0 + j) * 64 + lane);
       }
-      bu_probe(v, ok, hit, par, visited, row_off, col, first);
+      bu_first(v, ok, hit, visited, first, pred_new, level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         const uint64_t hm = __ballot(hit[j]);
         if (lane == j0 + j) newbits = hm;
+        const bool more = (readlane64(mymulti, j0 + j) >> lane) & 1ull;
+        fin[j] = ok[j] && !hit[j] && !more;
+        defer[j] = ok[j] && !hit[j] && more;
       }
-      wp += bu_commit(v, ok, hit, par, n2o, pred_new, level_bits, useg, wp);
+      wp += bu_append(v, fin, useg, wp, false);
+      dn += bu_append(v, defer, useg, kChunkVerts - dn, true);
     }
+    // deferred rows, lowest segment positions first: the front may then grow
+    // only over entries already read
+    fw[wave][lane] = 0;
+    wave_sync();
+    for (uint32_t r0 = 0; r0 < dn; r0 += kBuBatch) {
+      uint32_t v[kBuIlp];
+      bool ok[kBuIlp], hit[kBuIlp], miss[kBuIlp];
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) {
+        const uint32_t i = r0 + (uint32_t)(j * kWave + lane);
+        ok[j] = i < dn;
+        v[j] = ok[j] ? useg[kChunkVerts - dn + i] : vbase;
+      }
+      bu_rows(v, ok, hit, visited, row_off, col, n2o, pred_new, level_bits);
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) {
+        if (hit[j]) atomicOr(&fw[wave][(v[j] - vbase) >> 6], 1ull << (v[j] & 63));
+        miss[j] = ok[j] && !hit[j];
+      }
+      wp += bu_append(v, miss, useg, wp, false);
+    }
+    wave_sync();
+    newbits |= fw[wave][lane];
     if (w < words) vis_new[w] = myword | newbits;
     nfound += (uint64_t)__popcll(newbits);
     if (lane == 0) unv_cnt[c] = wp;
+    wave_sync();
   }
   const uint64_t s = block_sum(nfound, red);
   if (threadIdx.x == 0) {
@@ -477,10 +526,11 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
 __global__ __launch_bounds__(kT) void k_bu_list(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
-    const uint32_t* __restrict__ col, const uint32_t* __restrict__ first,
-    const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
-    uint32_t* __restrict__ unv, uint32_t* __restrict__ unv_cnt, LevelSync* ls,
-    HostReport* hr, unsigned long long seq, uint64_t level_bits) {
+    const uint32_t* __restrict__ col, const uint2* __restrict__ first,
+    const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
+    int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
+    uint32_t* __restrict__ unv_cnt, LevelSync* ls, HostReport* hr,
+    unsigned long long seq, uint64_t level_bits) {
   __shared__ unsigned long long fw[kWavesPerBlock][kWave];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ uint64_t hub[kHubWords];
@@ -499,20 +549,26 @@ __global__ __launch_bounds__(kT) void k_bu_list(
     wave_sync();
     // compaction in place: the write position never passes the batch read
     uint32_t wp = 0;
-    for (uint32_t e0 = 0; e0 < n; e0 += kWave * kBuIlp) {
-      uint32_t v[kBuIlp], par[kBuIlp];
-      bool ok[kBuIlp], hit[kBuIlp];
+    for (uint32_t e0 = 0; e0 < n; e0 += kBuBatch) {
+      uint32_t v[kBuIlp];
+      bool ok[kBuIlp], hit[kBuIlp], scan[kBuIlp], hit2[kBuIlp], miss[kBuIlp];
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         const uint32_t i = e0 + (uint32_t)(j * kWave + lane);
         ok[j] = i < n;
         v[j] = ok[j] ? useg[i] : vbase;
       }
-      bu_probe(v, ok, hit, par, visited, row_off, col, first);
+      bu_first(v, ok, hit, visited, first, pred_new, level_bits);
 #pragma unroll
-      for (int j = 0; j < kBuIlp; ++j)
+      for (int j = 0; j < kBuIlp; ++j) scan[j] = ok[j] && !hit[j] && multi_bit(multi, v[j]);
+      bu_rows(v, scan, hit2, visited, row_off, col, n2o, pred_new, level_bits);
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) {
+        hit[j] = hit[j] || hit2[j];
         if (hit[j]) atomicOr(&fw[wave][(v[j] - vbase) >> 6], 1ull << (v[j] & 63));
-      wp += bu_commit(v, ok, hit, par, n2o, pred_new, level_bits, useg, wp);
+        miss[j] = ok[j] && !hit[j];
+      }
+      wp += bu_append(v, miss, useg, wp, false);
     }
     wave_sync();
     const uint64_t fbits = fw[wave][lane];
@@ -691,12 +747,12 @@ void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_b
   if (lists)
     hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kT), 0, b->stream, b->vis.get(), b->vis_alt.get(),
                        b->words, g->row_off.get(), g->col.get(), g->first.get(),
-                       g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
+                       g->multi.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
                        b->lsync.get(), b->d_report, seq, level_bits);
   else
     hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kT), 0, b->stream, b->vis.get(), b->vis_alt.get(),
                        b->words, g->row_off.get(), g->col.get(), g->first.get(),
-                       g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
+                       g->multi.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
                        b->lsync.get(), b->d_report, seq, level_bits);
   G500_CHECK_LAUNCH();
   std::swap(b->vis.p, b->vis_alt.p);

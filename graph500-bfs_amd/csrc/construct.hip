@@ -207,15 +207,32 @@ __global__ __launch_bounds__(kT) void k_dedup_write(
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
-// Copy of each row's first entry, stored densely so that the bottom-up sweep
-// reads it coalesced (the reference keeps the same "isolated first edge",
-// mpi/graph_constructor.hpp:1387-1557).
+// Copy of each row's first entry with its original ID, stored densely so that
+// the bottom-up sweep reads both coalesced (the reference keeps the same
+// "isolated first edge", mpi/graph_constructor.hpp:1387-1557).
 __global__ __launch_bounds__(kT) void k_first_nbr(const int64_t* __restrict__ row_off,
                                                  const uint32_t* __restrict__ col,
-                                                 int64_t n_c,
-                                                 uint32_t* __restrict__ first) {
+                                                 const uint32_t* __restrict__ n2o,
+                                                 int64_t n_c, uint2* __restrict__ first) {
   const int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (v < n_c) first[v] = col[row_off[v]];
+  if (v < n_c) {
+    const uint32_t u = col[row_off[v]];
+    first[v] = make_uint2(u, n2o[u]);
+  }
+}
+
+// Bit v: row v has more than one entry (a bottom-up vertex whose first
+// neighbour is unvisited has nothing else to scan otherwise).
+__global__ __launch_bounds__(kT) void k_multi_bits(const int64_t* __restrict__ row_off,
+                                                  int64_t n_c, uint64_t* __restrict__ multi) {
+  const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (w * 64 >= n_c) return;
+  uint64_t bits = 0;
+  const int64_t v0 = w * 64;
+  const int cnt = (int)(n_c - v0 < 64 ? n_c - v0 : 64);
+  for (int b = 0; b < cnt; ++b)
+    if (row_off[v0 + b + 1] - row_off[v0 + b] > 1) bits |= 1ull << b;
+  multi[w] = bits;
 }
 
 int bitlen(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
@@ -322,9 +339,14 @@ void build_csr(g500_graph* g, const uint2* edges, int64_t ne, bool symmetric,
                        g->m);
     G500_CHECK_LAUNCH();
     g->first.alloc((size_t)std::max<int64_t>(g->n_c, 1));
+    g->multi.alloc((size_t)ceil_div(std::max<int64_t>(g->n_c, 1), 64));
     if (g->n_c > 0) {
       hipLaunchKernelGGL(k_first_nbr, dim3((unsigned)ceil_div(g->n_c, kT)), dim3(kT), 0, s,
-                         g->row_off.get(), g->col.get(), g->n_c, g->first.get());
+                         g->row_off.get(), g->col.get(), g->new2orig.get(), g->n_c,
+                         g->first.get());
+      G500_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_multi_bits, dim3((unsigned)ceil_div(ceil_div(g->n_c, 64), kT)),
+                         dim3(kT), 0, s, g->row_off.get(), g->n_c, g->multi.get());
       G500_CHECK_LAUNCH();
     }
     G500_CHECK_HIP(hipStreamSynchronize(s));
@@ -349,7 +371,8 @@ void ensure_host_orig2new(g500_graph* g) {
 
 int64_t g500_graph::device_bytes() const {
   return (int64_t)(edges.n * sizeof(uint2) + row_off.n * sizeof(int64_t) +
-                   (col.n + first.n) * sizeof(uint32_t) +
+                   col.n * sizeof(uint32_t) + first.n * sizeof(uint2) +
+                   multi.n * sizeof(uint64_t) +
                    (deg_new.n + new2orig.n + orig2new.n) * sizeof(uint32_t) +
                    tree_parent.n * sizeof(uint32_t) + has_parent.n * sizeof(uint64_t));
 }

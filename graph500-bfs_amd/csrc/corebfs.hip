@@ -298,6 +298,7 @@ void corebfs_prune(g500_graph* g) {
   // drop the full CSR before building the core one
   g->col.release();
   g->first.release();
+  g->multi.release();
   g->row_off.release();
   build_csr(g, core_edges.get(), (int64_t)m_core, true, log_n);
   core_edges.release();

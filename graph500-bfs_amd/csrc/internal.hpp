@@ -30,7 +30,8 @@ struct g500_graph {
   int64_t max_deg = 0;
   g500::DevBuf<int64_t> row_off;    // n_c + 1
   g500::DevBuf<uint32_t> col;       // m
-  g500::DevBuf<uint32_t> first;     // n_c: first (highest-degree) neighbour
+  g500::DevBuf<uint2> first;        // n_c: first (highest-degree) neighbour: new ID, original ID
+  g500::DevBuf<uint64_t> multi;     // n_c bits: the row has more than one entry
   g500::DevBuf<uint32_t> deg_new;   // n   (symmetric degree incl. duplicates)
   g500::DevBuf<uint32_t> new2orig;  // n
   g500::DevBuf<uint32_t> orig2new;  // n
