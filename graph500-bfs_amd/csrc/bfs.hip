@@ -466,17 +466,40 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
     // final misses fill the segment from the front, deferred row scans from
     // the back; together they never exceed the chunk's unvisited vertices
     uint32_t wp = 0, dn = 0;
-    // lane l takes bit l of words j0 .. j0+kBuIlp-1: coalesced `first` reads
+    // lane l takes bit l of words j0 .. j0+kBuIlp-1: coalesced `first`
+    // reads.  The next batch's pairs are requested before this batch's
+    // probes and stores, so their latency overlaps the work (vector memory
+    // counters retire in order, so loads issued after a store would also
+    // wait for it)
+    uint2 fn[kBuIlp];
+    bool okn[kBuIlp];
+#pragma unroll
+    for (int j = 0; j < kBuIlp; ++j) {
+      okn[j] = !((readlane64(myword, j) >> lane) & 1ull);
+      fn[j] = okn[j] ? first[vbase + (uint32_t)(j * 64 + lane)] : make_uint2(0u, 0u);
+    }
     for (int j0 = 0; j0 < kWave; j0 += kBuIlp) {
       uint32_t v[kBuIlp];
+      uint2 f[kBuIlp];
       bool ok[kBuIlp], hit[kBuIlp], fin[kBuIlp], defer[kBuIlp];
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
-        const uint64_t wd = readlane64(myword, j0 + j);
-        ok[j] = !((wd >> lane) & 1ull);
+        ok[j] = okn[j];
+        f[j] = fn[j];
         v[j] = vbase + (uint32_t)((j0 + j) * 64 + lane);
       }
-      bu_first(v, ok, hit, visited, first, pred_new, level_bits);
+      if (j0 + kBuIlp < kWave) {
+#pragma unroll
+        for (int j = 0; j < kBuIlp; ++j) {
+          okn[j] = !((readlane64(myword, j0 + kBuIlp + j) >> lane) & 1ull);
+          fn[j] = okn[j] ? first[v[j] + kBuIlp * 64] : make_uint2(0u, 0u);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(f[j].x);
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j)
+        if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)f[j].y | level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         const uint64_t hm = __ballot(hit[j]);
