@@ -9,7 +9,9 @@
 //
 // Kernels
 //   k_td_expand   top-down: one prefix sum over frontier degrees and a
-//                 tile -> owner table (k_td_tiles), then a persistent grid
+//                 tile -> owner table (k_td_prep_sums, k_td_prep_tiles; a
+//                 frontier of up to 512 vertices is staged by the expansion
+//                 itself), then a persistent grid
 //                 walks 2048-edge tiles; the tile's slice of the prefix sum
 //                 sits in LDS and each lane binary-searches its edges' owners
 //                 there (every lane gets eight edges, whatever the degree
@@ -143,17 +145,6 @@ __global__ void k_bfs_root(int64_t* pred_new, int64_t entry, uint32_t* q,
   }
 }
 
-__global__ __launch_bounds__(kT) void k_td_degrees(const uint32_t* __restrict__ q,
-                                                  int64_t qlen,
-                                                  const int64_t* __restrict__ row_off,
-                                                  uint64_t* __restrict__ dsum) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < qlen) {
-    const uint32_t u = q[i];
-    dsum[i] = (uint64_t)(row_off[u + 1] - row_off[u]);
-  }
-}
-
 // Filter before the visited-bit atomicOr (which alone decides the winner).
 __device__ inline unsigned long long probe_vis(unsigned long long* p) {
   return __atomic_load_n(p, __ATOMIC_RELAXED);
@@ -170,22 +161,99 @@ __device__ inline int64_t owner_search(const uint64_t* a, int64_t lo,
   return lo;
 }
 
-// tile_owner[t] = frontier index whose edge range holds edge t * kTile
-// (each tile start lies in exactly one range: frontier rows are non-empty);
-// the last vertex also writes the entry after the last tile.
-template <int kTile>
-__global__ __launch_bounds__(kT) void k_td_tiles(const uint64_t* __restrict__ dsum,
-                                                int64_t qlen,
-                                                uint32_t* __restrict__ tile_owner) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= qlen) return;
-  const uint64_t a = dsum[i], b = dsum[i + 1];
-  if (b <= a) return;
-  const uint64_t t0 = (a + kTile - 1) / kTile, t1 = (b - 1) / kTile;
-  for (uint64_t t = t0; t <= t1; ++t) tile_owner[t] = (uint32_t)i;
-  if (i == qlen - 1) tile_owner[t1 + 1] = (uint32_t)i;
+// Top-down preparation of a frontier of more than kTdWin vertices, in two
+// launches: dsum[i] = exclusive prefix sum of the frontier's degrees
+// (dsum[qlen] = total) and tile_owner[t] = frontier index whose edge range
+// holds edge t * kTdTile (the last vertex also writes the entry after the
+// last tile).  Block b owns entries [b * kPrepTile, (b + 1) * kPrepTile),
+// thread t of it kPrepItems consecutive ones.
+constexpr int kPrepItems = 8;
+constexpr int kPrepTile = kT * kPrepItems;
+
+// Block-local prefix sums into dsum, block totals into bsum; the last block
+// to finish (atomic ticket, which it resets) scans the totals into boff.
+__global__ __launch_bounds__(kT) void k_td_prep_sums(
+    const uint32_t* __restrict__ q, int64_t qlen, const int64_t* __restrict__ row_off,
+    uint64_t* __restrict__ dsum, uint64_t* __restrict__ bsum, uint64_t* __restrict__ boff,
+    unsigned int* ticket) {
+  __shared__ uint64_t wsum[kWavesPerBlock];
+  __shared__ bool last;
+  const int t = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * kPrepTile + (int64_t)t * kPrepItems;
+  uint32_t u[kPrepItems];
+#pragma unroll
+  for (int j = 0; j < kPrepItems; ++j) u[j] = i0 + j < qlen ? q[i0 + j] : 0u;
+  uint64_t d[kPrepItems];
+#pragma unroll
+  for (int j = 0; j < kPrepItems; ++j)
+    d[j] = i0 + j < qlen ? (uint64_t)(row_off[u[j] + 1] - row_off[u[j]]) : 0u;
+  uint64_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < kPrepItems; ++j) sum += d[j];
+  uint64_t tot;
+  uint64_t a = block_excl_scan(sum, wsum, &tot);
+#pragma unroll
+  for (int j = 0; j < kPrepItems; ++j) {
+    if (i0 + j < qlen) dsum[i0 + j] = a;
+    a += d[j];
+  }
+  if (t == 0) {
+    bsum[blockIdx.x] = tot;
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  uint64_t carry = 0;
+  const int64_t nb = gridDim.x;
+  for (int64_t b0 = 0; b0 < nb; b0 += kT) {
+    const uint64_t x = b0 + t < nb ? __atomic_load_n(&bsum[b0 + t], __ATOMIC_RELAXED) : 0;
+    uint64_t ctot;
+    const uint64_t ex = block_excl_scan(x, wsum, &ctot);
+    if (b0 + t < nb) boff[b0 + t] = carry + ex;
+    carry += ctot;
+  }
+  if (t == 0) {
+    dsum[qlen] = carry;
+    *ticket = 0u;
+  }
 }
 
+// Adds the block offsets and writes the tile owners.
+__global__ __launch_bounds__(kT) void k_td_prep_tiles(
+    int64_t qlen, uint64_t* __restrict__ dsum, const uint64_t* __restrict__ bsum,
+    const uint64_t* __restrict__ boff, uint32_t* __restrict__ tile_owner) {
+  const int t = threadIdx.x;
+  const int64_t blk = blockIdx.x;
+  const int64_t i0 = blk * kPrepTile + (int64_t)t * kPrepItems;
+  const int64_t bend = min((blk + 1) * (int64_t)kPrepTile, qlen);
+  // local prefix sums of this thread's entries and of the entry after them
+  // (the next thread's first, or the block total), read before any write
+  uint64_t loc[kPrepItems + 1];
+#pragma unroll
+  for (int j = 0; j <= kPrepItems; ++j) {
+    const int64_t i = i0 + j;
+    loc[j] = i < bend ? dsum[i] : bsum[blk];
+  }
+  const uint64_t off = boff[blk];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPrepItems; ++j) {
+    const int64_t i = i0 + j;
+    if (i >= bend) break;
+    const uint64_t a = off + loc[j], b = off + loc[j + 1];
+    dsum[i] = a;
+    if (b <= a) continue;
+    const uint64_t t0 = (a + kTdTile - 1) / kTdTile, t1 = (b - 1) / kTdTile;
+    for (uint64_t tt = t0; tt <= t1; ++tt) tile_owner[tt] = (uint32_t)i;
+    if (i == qlen - 1) tile_owner[t1 + 1] = (uint32_t)i;
+  }
+}
+
+// Frontiers of at most kTdWin vertices need no preparation kernels: every
+// block stages the whole frontier (prefix sum of degrees, row base, original
+// ID) in LDS itself.
 __global__ __launch_bounds__(kT) void k_td_expand(
     const uint32_t* __restrict__ q, int64_t qlen,
     const uint64_t* __restrict__ dsum, const uint32_t* __restrict__ tile_owner,
@@ -209,7 +277,39 @@ __global__ __launch_bounds__(kT) void k_td_expand(
   __shared__ unsigned long long lq_base;
 
   const int t = threadIdx.x;
-  const uint64_t E = dsum[qlen];
+  const bool small = qlen <= kTdWin;
+  uint64_t E;
+  if (small) {
+    // lane t stages frontier entries 2t and 2t+1
+    const int64_t i0 = 2 * t, i1 = 2 * t + 1;
+    uint32_t u0 = 0, u1 = 0;
+    int64_t r0 = 0, r1 = 0;
+    uint64_t d0 = 0, d1 = 0;
+    if (i0 < qlen) {
+      u0 = q[i0];
+      r0 = row_off[u0];
+      d0 = (uint64_t)(row_off[u0 + 1] - r0);
+    }
+    if (i1 < qlen) {
+      u1 = q[i1];
+      r1 = row_off[u1];
+      d1 = (uint64_t)(row_off[u1 + 1] - r1);
+    }
+    const uint64_t ex = block_excl_scan(d0 + d1, red, &E);
+    if (i0 < qlen) {
+      win[i0] = ex;
+      wrb[i0] = r0 - (int64_t)ex;
+      wpu[i0] = n2o[u0];
+    }
+    if (i1 < qlen) {
+      win[i1] = ex + d0;
+      wrb[i1] = r1 - (int64_t)(ex + d0);
+      wpu[i1] = n2o[u1];
+    }
+    __syncthreads();
+  } else {
+    E = dsum[qlen];
+  }
   uint64_t my_edges = 0;
   for (uint64_t e0 = (uint64_t)blockIdx.x * kTdTile; e0 < E;
        e0 += (uint64_t)gridDim.x * kTdTile) {
@@ -218,15 +318,15 @@ __global__ __launch_bounds__(kT) void k_td_expand(
       // owners of the tile's first edge and of the next tile's first edge
       // (>= the owner of its last edge)
       const uint64_t tile = e0 / kTdTile;
-      s_lo = tile_owner[tile];
-      s_hi = min((int64_t)tile_owner[tile + 1], qlen - 1);
+      s_lo = small ? 0 : tile_owner[tile];
+      s_hi = small ? qlen - 1 : min((int64_t)tile_owner[tile + 1], qlen - 1);
       lq_n = 0;
     }
     __syncthreads();
     const int64_t lo = s_lo, hi = s_hi;
     const int64_t wn = hi - lo + 1;
     const bool use_win = wn <= kTdWin;
-    if (use_win)
+    if (use_win && !small)
       for (int64_t j = t; j < wn; j += kT) {
         const uint64_t d = dsum[lo + j];
         const uint32_t u = q[lo + j];
@@ -700,6 +800,10 @@ void bfs_init(g500_bfs* b) {
   b->dsum.alloc((size_t)std::max<int64_t>(g->n_c, 1) + 1);
   // a frontier's edges never exceed m
   b->tile_owner.alloc((size_t)ceil_div(std::max<int64_t>(g->m, 1), kTdTile) + 2);
+  b->bsum.alloc((size_t)ceil_div(std::max<int64_t>(g->n_c, 1), kPrepTile));
+  b->boff.alloc((size_t)ceil_div(std::max<int64_t>(g->n_c, 1), kPrepTile));
+  b->prep_ticket.alloc(1);
+  G500_CHECK_HIP(hipMemset(b->prep_ticket.get(), 0, sizeof(unsigned int)));
   const int64_t chunks = ceil_div(b->words, kWave);
   b->unv.alloc((size_t)(chunks * kChunkVerts));
   b->unv_cnt.alloc((size_t)chunks);
@@ -717,7 +821,6 @@ void bfs_init(g500_bfs* b) {
                              ceil_div(chunks, kWavesPerBlock)));
   b->lsync.alloc(1);
   G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
-  b->scan_ws.reserve(std::max<int64_t>(g->n_c, 1));
   G500_CHECK_HIP(hipHostMalloc((void**)&b->h_report, sizeof(HostReport),
                                hipHostMallocMapped | hipHostMallocCoherent));
   std::memset((void*)b->h_report, 0, sizeof(HostReport));
@@ -879,15 +982,15 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                          b->nq.get(), ls, b->d_report, seq);
       G500_CHECK_LAUNCH();
     } else if (td) {
-      if (qlen > 0) {
-        hipLaunchKernelGGL(k_td_degrees, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT),
-                           0, s, b->q.get(), qlen, g->row_off.get(), b->dsum.get());
+      // small frontiers: the expansion stages them itself
+      if (qlen > kTdWin) {
+        const dim3 pg((unsigned)ceil_div(qlen, kPrepTile));
+        hipLaunchKernelGGL(k_td_prep_sums, pg, dim3(kT), 0, s, b->q.get(), qlen,
+                           g->row_off.get(), b->dsum.get(), b->bsum.get(), b->boff.get(),
+                           b->prep_ticket.get());
         G500_CHECK_LAUNCH();
-      }
-      exclusive_scan<uint64_t>(b->dsum.get(), b->dsum.get(), qlen, b->scan_ws, s);
-      if (qlen > 0) {
-        hipLaunchKernelGGL(k_td_tiles<kTdTile>, dim3((unsigned)ceil_div(qlen, kT)), dim3(kT),
-                           0, s, b->dsum.get(), qlen, b->tile_owner.get());
+        hipLaunchKernelGGL(k_td_prep_tiles, pg, dim3(kT), 0, s, qlen, b->dsum.get(),
+                           b->bsum.get(), b->boff.get(), b->tile_owner.get());
         G500_CHECK_LAUNCH();
       }
       const unsigned grid = est_edges < 0 ? 256u : grid_for(est_edges, kTdTile);

@@ -30,6 +30,8 @@ struct g500_bfs {
   g500::DevBuf<uint32_t> q, nq;     // frontier queues (new IDs)
   g500::DevBuf<uint64_t> dsum;      // frontier degree prefix sums
   g500::DevBuf<uint32_t> tile_owner; // frontier index owning each top-down tile's first edge
+  g500::DevBuf<uint64_t> bsum, boff;  // per prep block: degree total, its offset
+  g500::DevBuf<unsigned int> prep_ticket;  // blocks done in the prep's first launch
   // unvisited vertices after a bottom-up level: 4096 entries per chunk of 64
   // bitmap words, unv_cnt[chunk] of them in use
   g500::DevBuf<uint32_t> unv, unv_cnt;
@@ -38,7 +40,6 @@ struct g500_bfs {
   HostReport* h_report = nullptr;  // pinned host memory, written by the GPU
   HostReport* d_report = nullptr;  // its device address
   unsigned long long seq = 0;      // last report sequence number issued
-  g500::ScanWorkspace scan_ws;
 
   // validation scratch
   g500::DevBuf<unsigned long long> vcount;  // [G500_VERR_NUM + 2]

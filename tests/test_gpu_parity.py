@@ -104,6 +104,19 @@ def test_direction_extremes(G, alpha, beta):
     _check_roots(G, g, og, g.find_roots(3), alpha=alpha, beta=beta)
 
 
+def test_top_down_frontier_sizes(G):
+    """Pure top-down at SCALE 18: frontiers below 512 vertices (staged by
+    the expansion itself), up to 64K (one prep block) and beyond (degree
+    scan and tile owners over the whole grid) all occur."""
+    print("\n[dir] scale=18 pure top-down", flush=True)
+    g = G.Graph.generate(18).construct()
+    o0, o1 = O.generate_edges(18)
+    og = O.Graph(o0, o1)
+    bfs = _check_roots(G, g, og, g.find_roots(2), alpha=1e9, beta=14.0)
+    sizes = [t["nq_size"] for t in bfs.stats()[1]]
+    assert max(sizes) > 65536 and any(512 < s <= 65536 for s in sizes)
+
+
 def _small_cases():
     rng = np.random.default_rng(5)
     path = np.arange(9)
