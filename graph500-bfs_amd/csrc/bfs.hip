@@ -479,8 +479,11 @@ __device__ inline void bu_first(const uint32_t (&v)[kBuIlp], const bool (&ok)[kB
 
 // Scans the rows of scan[j] vertices past their first entry; hits get their
 // pred entries (first visited neighbour in row order).
+// Rows with more than max_rest entries left are not scanned: long[j] is set
+// for them instead.
 __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[kBuIlp],
-                               bool (&hit)[kBuIlp], const VisProbe& visited,
+                               bool (&hit)[kBuIlp], bool (&lng)[kBuIlp],
+                               int64_t max_rest, const VisProbe& visited,
                                const int64_t* __restrict__ row_off,
                                const uint32_t* __restrict__ col,
                                const uint32_t* __restrict__ n2o,
@@ -490,6 +493,8 @@ __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[k
   for (int j = 0; j < kBuIlp; ++j) {
     k[j] = scan[j] ? row_off[v[j]] + 1 : 0;
     k1[j] = scan[j] ? row_off[v[j] + 1] : 0;
+    lng[j] = k1[j] - k[j] > max_rest;
+    if (lng[j]) k1[j] = k[j];
   }
   uint32_t par[kBuIlp];
 #pragma unroll
@@ -540,14 +545,21 @@ __device__ inline bool multi_bit(const uint64_t* multi, uint32_t v) {
   return (multi[v >> 6] >> (v & 63)) & 1ull;
 }
 
+// Rows with more than kBuLongRow entries left after a missed first
+// neighbour are not scanned by k_bu_sweep's lanes: they are listed for
+// k_bu_long, which gives each one a whole wave (one such row would otherwise
+// hold its wave far longer than the rest).  k_bu_long closes the level.
+constexpr int64_t kBuLongRow = 256;
+constexpr int kBuStep = 8;  // row entries a lane probes per step
+
 __global__ __launch_bounds__(kT) void k_bu_sweep(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
-    uint32_t* __restrict__ unv_cnt, LevelSync* ls, HostReport* hr,
-    unsigned long long seq, uint64_t level_bits) {
+    uint32_t* __restrict__ unv_cnt, uint32_t* __restrict__ longv, LevelSync* ls,
+    uint64_t level_bits) {
   __shared__ unsigned long long fw[kWavesPerBlock][kWave];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ uint64_t hub[kHubWords];
@@ -611,26 +623,72 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
       wp += bu_append(v, fin, useg, wp, false);
       dn += bu_append(v, defer, useg, kChunkVerts - dn, true);
     }
-    // deferred rows, lowest segment positions first: the front may then grow
-    // only over entries already read
+    // deferred rows: a lane takes the next one (lowest segment position
+    // first) whenever it has none, and scans kBuStep entries per step, so a
+    // wave's time follows its total work rather than its longest row per
+    // batch.  The front grows only over entries already read (each row
+    // finished was taken before).  Rows longer than kBuLongRow go to
+    // k_bu_long.
     fw[wave][lane] = 0;
     wave_sync();
-    for (uint32_t r0 = 0; r0 < dn; r0 += kBuBatch) {
-      uint32_t v[kBuIlp];
-      bool ok[kBuIlp], hit[kBuIlp], miss[kBuIlp];
+    {
+      const uint32_t dbase = kChunkVerts - dn;
+      uint32_t taken = 0;  // wave-uniform
+      bool have = false;
+      uint32_t dv = 0;
+      int64_t dk = 0, dk1 = 0;
+      const uint64_t below = lanemask_lt();
+      for (;;) {
+        const uint64_t idle = __ballot(!have);
+        const uint32_t idx = taken + (uint32_t)__popcll(idle & below);
+        taken += (uint32_t)__popcll(idle);
+        if (taken > dn) taken = dn;
+        bool lng = false;
+        if (!have && idx < dn) {
+          dv = useg[dbase + idx];
+          dk = row_off[dv] + 1;
+          dk1 = row_off[dv + 1];
+          have = true;
+          lng = dk1 - dk > kBuLongRow;
+        }
+        if (__ballot(have) == 0) break;
+        const uint64_t lm = __ballot(lng);
+        if (lm) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(&ls->cnt[2], (unsigned long long)__popcll(lm));
+          base = __shfl(base, 0, kWave);
+          if (lng) {
+            longv[base + (uint64_t)__popcll(lm & below)] = dv;
+            have = false;
+          }
+        }
+        bool hit = false, miss = false;
+        uint32_t par = 0;
+        if (have) {
+          uint32_t u[kBuStep];
 #pragma unroll
-      for (int j = 0; j < kBuIlp; ++j) {
-        const uint32_t i = r0 + (uint32_t)(j * kWave + lane);
-        ok[j] = i < dn;
-        v[j] = ok[j] ? useg[kChunkVerts - dn + i] : vbase;
-      }
-      bu_rows(v, ok, hit, visited, row_off, col, n2o, pred_new, level_bits);
+          for (int t = 0; t < kBuStep; ++t) u[t] = dk + t < dk1 ? col[dk + t] : 0u;
+          bool h[kBuStep];
 #pragma unroll
-      for (int j = 0; j < kBuIlp; ++j) {
-        if (hit[j]) atomicOr(&fw[wave][(v[j] - vbase) >> 6], 1ull << (v[j] & 63));
-        miss[j] = ok[j] && !hit[j];
+          for (int t = 0; t < kBuStep; ++t) h[t] = dk + t < dk1 && visited(u[t]);
+#pragma unroll
+          for (int t = kBuStep - 1; t >= 0; --t)
+            if (h[t]) {
+              hit = true;
+              par = u[t];
+            }
+          dk += kBuStep;
+          miss = !hit && dk >= dk1;
+        }
+        if (hit) {
+          pred_new[dv] = (int64_t)((uint64_t)n2o[par] | level_bits);
+          atomicOr(&fw[wave][(dv - vbase) >> 6], 1ull << (dv & 63));
+        }
+        const uint64_t mm = __ballot(miss);
+        if (miss) useg[wp + (uint32_t)__popcll(mm & below)] = dv;
+        wp += (uint32_t)__popcll(mm);
+        if (hit || miss) have = false;
       }
-      wp += bu_append(v, miss, useg, wp, false);
     }
     wave_sync();
     newbits |= fw[wave][lane];
@@ -638,6 +696,65 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
     nfound += (uint64_t)__popcll(newbits);
     if (lane == 0) unv_cnt[c] = wp;
     wave_sync();
+  }
+  const uint64_t s = block_sum(nfound, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&ls->cnt[0], (unsigned long long)s);
+}
+
+// The rows k_bu_sweep listed, a wave each: 256 entries per step, first
+// visited neighbour in row order; a vertex without one joins its chunk's
+// unvisited list.
+__global__ __launch_bounds__(kT) void k_bu_long(
+    const uint64_t* __restrict__ vis_old, unsigned long long* vis_new, int64_t words,
+    const int64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+    const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
+    uint32_t* __restrict__ unv, uint32_t* unv_cnt,
+    const uint32_t* __restrict__ longv, LevelSync* ls, HostReport* hr,
+    unsigned long long seq, uint64_t level_bits) {
+  __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ uint64_t hub[kHubWords];
+  const VisProbe visited = stage_hubs(hub, vis_old, words);
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int64_t n_long = (int64_t)ls->cnt[2];
+  uint64_t nfound = 0;
+  for (int64_t it = (int64_t)blockIdx.x * kWavesPerBlock + wave; it < n_long;
+       it += (int64_t)gridDim.x * kWavesPerBlock) {
+    const uint32_t v = longv[it];
+    int64_t k = row_off[v] + 1;
+    const int64_t k1 = row_off[v + 1];
+    bool found = false;
+    uint32_t par = 0;
+    while (!found && k < k1) {
+      uint32_t u[4];
+      bool h[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t e = k + i * kWave + lane;
+        u[i] = e < k1 ? col[e] : 0u;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[i] = k + i * kWave + lane < k1 && visited(u[i]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint64_t m = __ballot(h[i]);
+        if (m && !found) {
+          found = true;
+          par = __shfl(u[i], __ffsll((long long)m) - 1, kWave);
+        }
+      }
+      k += 4 * kWave;
+    }
+    if (lane == 0) {
+      if (found) {
+        pred_new[v] = (int64_t)((uint64_t)n2o[par] | level_bits);
+        atomicOr(vis_new + (v >> 6), 1ull << (v & 63));
+        ++nfound;
+      } else {
+        const uint32_t c = v / (uint32_t)kChunkVerts;
+        const uint32_t pos = atomicAdd(unv_cnt + c, 1u);
+        unv[(int64_t)c * kChunkVerts + pos] = v;
+      }
+    }
   }
   const uint64_t s = block_sum(nfound, red);
   if (threadIdx.x == 0) {
@@ -684,7 +801,8 @@ __global__ __launch_bounds__(kT) void k_bu_list(
       bu_first(v, ok, hit, visited, first, pred_new, level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) scan[j] = ok[j] && !hit[j] && multi_bit(multi, v[j]);
-      bu_rows(v, scan, hit2, visited, row_off, col, n2o, pred_new, level_bits);
+      bool lng[kBuIlp];
+      bu_rows(v, scan, hit2, lng, INT64_MAX, visited, row_off, col, n2o, pred_new, level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         hit[j] = hit[j] || hit2[j];
@@ -708,29 +826,65 @@ __global__ __launch_bounds__(kT) void k_bu_list(
 }
 
 // queue <- bits set in cur but not in prev; reports (count, degree sum).
+// Block b owns a contiguous range of 64-word groups: one pass counts its new
+// vertices and reserves their queue range with a single atomic (same-address
+// atomics serialize, so the grid stays small), the second takes the groups
+// one by one: thread t compacts quarter t % 4 of word t / 4 into LDS (at most
+// 4096 vertices per group), then the block writes its slice coalesced and
+// loads the degrees in parallel.
+constexpr int kQueueWords = kT / 4;
+constexpr int kQueueGrid = 1024;
+
 __global__ __launch_bounds__(kT) void k_bitmap_to_queue(
     const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
     int64_t words, const uint32_t* __restrict__ deg_new,
     uint32_t* __restrict__ out, LevelSync* ls, HostReport* hr,
     unsigned long long seq) {
+  __shared__ uint32_t stage[kQueueWords * 64];
   __shared__ uint64_t wsum[kWavesPerBlock];
   __shared__ unsigned long long base;
-  const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
-  uint64_t bits = w < words ? (cur[w] & ~prev[w]) : 0;
-  uint64_t tot;
-  const uint64_t excl = block_excl_scan((uint64_t)__popcll(bits), wsum, &tot);
-  if (threadIdx.x == 0) base = tot ? atomicAdd(&ls->cnt[0], (unsigned long long)tot) : 0;
+  const int t = threadIdx.x;
+  const int q = (t & 3) * 16;
+  const int64_t groups = ceil_div(words, kQueueWords);
+  const int64_t per = ceil_div(groups, (int64_t)gridDim.x);
+  const int64_t g0 = (int64_t)blockIdx.x * per;
+  const int64_t g1 = min(g0 + per, groups);
+  auto quarter = [&](int64_t g) -> uint32_t {
+    const int64_t w = g * kQueueWords + t / 4;
+    return w < words ? (uint32_t)(((cur[w] & ~prev[w]) >> q) & 0xFFFFu) : 0u;
+  };
+  uint64_t mine = 0;
+  for (int64_t g = g0; g < g1; ++g) mine += (uint64_t)__popc(quarter(g));
+  const uint64_t all = block_sum(mine, wsum);
+  if (t == 0) base = all ? atomicAdd(&ls->cnt[0], (unsigned long long)all) : 0;
   __syncthreads();
-  uint64_t pos = base + excl, edges = 0;
-  while (bits) {
-    const int b = __ffsll((long long)bits) - 1;
-    bits &= bits - 1;
-    const uint32_t v = (uint32_t)(w * 64 + b);
-    out[pos++] = v;
-    edges += deg_new[v];
+  uint64_t off = base, edges = 0;
+  for (int64_t g = g0; g < g1; ++g) {
+    uint32_t bits = quarter(g);
+    uint64_t tot;
+    uint32_t pos = (uint32_t)block_excl_scan((uint64_t)__popc(bits), wsum, &tot);
+    const int64_t w = g * kQueueWords + t / 4;
+    while (bits) {
+      const int bit = __ffs(bits) - 1;
+      bits &= bits - 1;
+      stage[pos++] = (uint32_t)(w * 64 + q + bit);
+    }
+    __syncthreads();
+    const uint32_t n = (uint32_t)tot;
+#pragma unroll
+    for (int r = 0; r < kQueueWords * 64 / kT; ++r) {
+      const uint32_t i = (uint32_t)(r * kT + t);
+      if (i < n) {
+        const uint32_t v = stage[i];
+        out[off + i] = v;
+        edges += deg_new[v];
+      }
+    }
+    off += tot;
+    __syncthreads();
   }
   const uint64_t s = block_sum(edges, wsum);
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     if (s) atomicAdd(&ls->cnt[1], (unsigned long long)s);
     finish_level(ls, hr, seq);
   }
@@ -809,13 +963,15 @@ void bfs_init(g500_bfs* b) {
   b->unv_cnt.alloc((size_t)chunks);
   // bottom-up grids: what stays resident (each block stages the hub bits
   // once), never more blocks than chunk groups
-  int cus = 0, per_cu[2] = {0, 0};
+  int cus = 0, per_cu[3] = {0, 0, 0};
   G500_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
   G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu[0], reinterpret_cast<const void*>(&k_bu_sweep), kT, 0));
   G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu[1], reinterpret_cast<const void*>(&k_bu_list), kT, 0));
-  for (int i = 0; i < 2; ++i)
+  G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu[2], reinterpret_cast<const void*>(&k_bu_long), kT, 0));
+  for (int i = 0; i < 3; ++i)
     b->bu_grid[i] = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)std::max(cus, 1) * std::max(per_cu[i], 1),
                              ceil_div(chunks, kWavesPerBlock)));
@@ -868,19 +1024,31 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
 
 // One bottom-up level: vis -> vis_alt, then the two swap.  `lists` walks
 // the unvisited lists the previous bottom-up level left, else the bitmap.
+// The sweep lists its long rows in nq, which no bottom-up level uses (a
+// top-down level run as a sweep refills it from the bitmap afterwards).
 void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_bits) {
   g500_graph* g = b->g;
-  if (lists)
-    hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kT), 0, b->stream, b->vis.get(), b->vis_alt.get(),
-                       b->words, g->row_off.get(), g->col.get(), g->first.get(),
-                       g->multi.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
-                       b->lsync.get(), b->d_report, seq, level_bits);
-  else
-    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kT), 0, b->stream, b->vis.get(), b->vis_alt.get(),
-                       b->words, g->row_off.get(), g->col.get(), g->first.get(),
-                       g->multi.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(), b->unv_cnt.get(),
-                       b->lsync.get(), b->d_report, seq, level_bits);
-  G500_CHECK_LAUNCH();
+  hipStream_t s = b->stream;
+  if (lists) {
+    hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kT), 0, s, b->vis.get(),
+                       b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
+                       g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
+                       b->unv.get(), b->unv_cnt.get(), b->lsync.get(), b->d_report, seq,
+                       level_bits);
+    G500_CHECK_LAUNCH();
+  } else {
+    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kT), 0, s, b->vis.get(),
+                       b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
+                       g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
+                       b->unv.get(), b->unv_cnt.get(), b->nq.get(), b->lsync.get(), level_bits);
+    G500_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kT), 0, s, b->vis.get(),
+                       (unsigned long long*)b->vis_alt.get(), b->words, g->row_off.get(),
+                       g->col.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
+                       b->unv_cnt.get(), b->nq.get(), b->lsync.get(), b->d_report, seq,
+                       level_bits);
+    G500_CHECK_LAUNCH();
+  }
   std::swap(b->vis.p, b->vis_alt.p);
 }
 
@@ -977,7 +1145,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       launch_bu(b, false, seq, level_bits);
       unv_ok = true;
       seq = ++b->seq;
-      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)ceil_div(words, kT)), dim3(kT),
+      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
                          0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
                          b->nq.get(), ls, b->d_report, seq);
       G500_CHECK_LAUNCH();
@@ -1049,7 +1217,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     } else if (next_td) {
       // switching to top-down: frontier = bits new in this level
       const unsigned long long cseq = ++b->seq;
-      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)ceil_div(words, kT)), dim3(kT),
+      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
                          0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
                          b->q.get(), ls, b->d_report, cseq);
       G500_CHECK_LAUNCH();

@@ -35,7 +35,7 @@ struct g500_bfs {
   // unvisited vertices after a bottom-up level: 4096 entries per chunk of 64
   // bitmap words, unv_cnt[chunk] of them in use
   g500::DevBuf<uint32_t> unv, unv_cnt;
-  unsigned bu_grid[2] = {1, 1};     // resident blocks of the sweep / list kernels
+  unsigned bu_grid[3] = {1, 1, 1};  // resident blocks of the sweep / list / long-row kernels
   g500::DevBuf<LevelSync> lsync;   // per-level counters
   HostReport* h_report = nullptr;  // pinned host memory, written by the GPU
   HostReport* d_report = nullptr;  // its device address
@@ -48,7 +48,7 @@ struct g500_bfs {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
   bool timing = false;
-  double exec_alpha = 0.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
+  double exec_alpha = 15.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
 
   bool have_result = false;
   int64_t last_root = -1;

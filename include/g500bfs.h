@@ -206,7 +206,8 @@ int g500_bfs_set_timing(g500_bfs* b, int on);
  * the rule runs top-down is executed as a bottom-up sweep when its frontier
  * holds more than m / exec_alpha edges (same vertices, same depths, any valid
  * parents).  0 disables (every top-down level runs top-down).  The default
- * comes from G500_EXEC_ALPHA in the environment, else 0. */
+ * comes from G500_EXEC_ALPHA in the environment, else 15 (on MI355X a
+ * bottom-up sweep beats the top-down expansion of such levels). */
 int g500_bfs_set_exec_alpha(g500_bfs* b, double exec_alpha);
 int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st,
                        g500_level_trace* levels, int max_levels);

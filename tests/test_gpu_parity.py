@@ -105,9 +105,10 @@ def test_direction_extremes(G, alpha, beta):
 
 
 def test_top_down_frontier_sizes(G):
-    """Pure top-down at SCALE 18: frontiers below 512 vertices (staged by
-    the expansion itself), up to 64K (one prep block) and beyond (degree
-    scan and tile owners over the whole grid) all occur."""
+    """Pure top-down at SCALE 18: frontiers of at most 512 vertices (staged
+    by the expansion itself) and larger ones (block prefix sums, a last-block
+    scan of up to 32 and of more than 32 block totals, then tile owners) all
+    occur."""
     print("\n[dir] scale=18 pure top-down", flush=True)
     g = G.Graph.generate(18).construct()
     o0, o1 = O.generate_edges(18)
@@ -206,16 +207,18 @@ def test_validator_agrees_on_planted_mistakes(G):
     assert ok and te == 65535
 
 
-@pytest.mark.parametrize("exec_alpha", [0.0, 1e9])
+@pytest.mark.parametrize("exec_alpha", [0.0, 1e9, None])
 def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
-    """Top-down levels run as top-down (0) or as bottom-up sweeps (1e9): the
-    levels, the direction schedule and auto_tuning_data stay identical."""
+    """Top-down levels run as top-down (0), as bottom-up sweeps (1e9) or as
+    the default decides (over m/15 edges: bottom-up): the levels, the
+    direction schedule and auto_tuning_data stay identical."""
     print(f"\n[exec] alpha_exec={exec_alpha}", flush=True)
     g = G.Graph.generate(16).construct()
     o0, o1 = O.generate_edges(16)
     og = O.Graph(o0, o1)
     bfs = G.Bfs(g)
-    bfs.set_exec_alpha(exec_alpha)
+    if exec_alpha is not None:
+        bfs.set_exec_alpha(exec_alpha)
     for r in g.find_roots(6):
         r = int(r)
         at = bfs.run(r)
@@ -228,5 +231,11 @@ def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
         assert at.tolist() == at_o.tolist()
         if exec_alpha == 0.0:
             assert all(t["exec_bottom_up"] == (0 if t["top_down"] else 1) for t in trace)
+        elif exec_alpha is None:
+            m = g.info.m
+            for prev, t in zip(trace, trace[1:]):
+                if t["top_down"]:
+                    heavy = prev["top_down"] and prev["nq_edges"] > m / 15
+                    assert t["exec_bottom_up"] == int(heavy), (r, t)
         else:
             assert all(t["exec_bottom_up"] == 1 for t in trace[1:])
