@@ -462,23 +462,6 @@ __device__ inline uint64_t readlane64(uint64_t x, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// First-neighbour probes of a batch; hits get their pred entries.
-__device__ inline void bu_first(const uint32_t (&v)[kBuIlp], const bool (&ok)[kBuIlp],
-                                bool (&hit)[kBuIlp], const VisProbe& visited,
-                                const uint2* __restrict__ first,
-                                int64_t* __restrict__ pred_new, uint64_t level_bits) {
-  uint2 f[kBuIlp];
-#pragma unroll
-  for (int j = 0; j < kBuIlp; ++j) f[j] = ok[j] ? first[v[j]] : make_uint2(0u, 0u);
-#pragma unroll
-  for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(f[j].x);
-#pragma unroll
-  for (int j = 0; j < kBuIlp; ++j)
-    if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)f[j].y | level_bits);
-}
-
-// Scans the rows of scan[j] vertices past their first entry; hits get their
-// pred entries (first visited neighbour in row order).
 // Rows with more than max_rest entries left are not scanned: long[j] is set
 // for them instead.
 __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[kBuIlp],
@@ -787,18 +770,47 @@ __global__ __launch_bounds__(kT) void k_bu_list(
     const uint32_t vbase = (uint32_t)(c * kChunkVerts);
     fw[wave][lane] = 0;
     wave_sync();
-    // compaction in place: the write position never passes the batch read
+    // compaction in place: misses of a batch land below the next batch, so
+    // the write position never passes an entry not yet read.  The next
+    // batch's entries and first-neighbour pairs are requested before this
+    // batch's probes and stores (as in the sweep).
     uint32_t wp = 0;
+    uint32_t vn[kBuIlp];
+    uint2 fn[kBuIlp];
+#pragma unroll
+    for (int j = 0; j < kBuIlp; ++j) {
+      const uint32_t i = (uint32_t)(j * kWave + lane);
+      vn[j] = i < n ? useg[i] : vbase;
+    }
+#pragma unroll
+    for (int j = 0; j < kBuIlp; ++j)
+      fn[j] = (uint32_t)(j * kWave + lane) < n ? first[vn[j]] : make_uint2(0u, 0u);
     for (uint32_t e0 = 0; e0 < n; e0 += kBuBatch) {
       uint32_t v[kBuIlp];
+      uint2 f[kBuIlp];
       bool ok[kBuIlp], hit[kBuIlp], scan[kBuIlp], hit2[kBuIlp], miss[kBuIlp];
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
-        const uint32_t i = e0 + (uint32_t)(j * kWave + lane);
-        ok[j] = i < n;
-        v[j] = ok[j] ? useg[i] : vbase;
+        ok[j] = e0 + (uint32_t)(j * kWave + lane) < n;
+        v[j] = vn[j];
+        f[j] = fn[j];
       }
-      bu_first(v, ok, hit, visited, first, pred_new, level_bits);
+      if (e0 + kBuBatch < n) {
+#pragma unroll
+        for (int j = 0; j < kBuIlp; ++j) {
+          const uint32_t i = e0 + kBuBatch + (uint32_t)(j * kWave + lane);
+          vn[j] = i < n ? useg[i] : vbase;
+        }
+#pragma unroll
+        for (int j = 0; j < kBuIlp; ++j)
+          fn[j] = e0 + kBuBatch + (uint32_t)(j * kWave + lane) < n ? first[vn[j]]
+                                                                   : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(f[j].x);
+#pragma unroll
+      for (int j = 0; j < kBuIlp; ++j)
+        if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)f[j].y | level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) scan[j] = ok[j] && !hit[j] && multi_bit(multi, v[j]);
       bool lng[kBuIlp];
