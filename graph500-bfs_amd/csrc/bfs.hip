@@ -145,6 +145,18 @@ __global__ void k_bfs_root(int64_t* pred_new, int64_t entry, uint32_t* q,
   }
 }
 
+// Streams that no kernel of the BFS reads back soon (pred entries, the
+// first-neighbour pairs, the unvisited lists) bypass the L2 with
+// non-temporal accesses, which leaves it to the visited bitmap's probes.
+__device__ inline void put_pred(int64_t* p, uint64_t entry) {
+  __builtin_nontemporal_store((int64_t)entry, p);
+}
+__device__ inline uint2 get_pair(const uint2* p) {
+  const unsigned long long x =
+      __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(p));
+  return make_uint2((uint32_t)x, (uint32_t)(x >> 32));
+}
+
 // Filter before the visited-bit atomicOr (which alone decides the winner).
 __device__ inline unsigned long long probe_vis(unsigned long long* p) {
   return __atomic_load_n(p, __ATOMIC_RELAXED);
@@ -379,7 +391,7 @@ __global__ __launch_bounds__(kT) void k_td_expand(
 #pragma unroll
     for (int j = 0; j < kTdItems; ++j)
       if (won[j]) {
-        pred_new[v[j]] = (int64_t)((uint64_t)pu[j] | level_bits);
+        put_pred(pred_new + v[j], (uint64_t)pu[j] | level_bits);
         my_edges += deg_new[v[j]];
       }
     // append winners to the block's LDS queue: wave prefix + one LDS atomic
@@ -505,7 +517,7 @@ __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[k
   for (int j = 0; j < kBuIlp; ++j) pp[j] = hit[j] ? n2o[par[j]] : 0u;
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j)
-    if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)pp[j] | level_bits);
+    if (hit[j]) put_pred(pred_new + v[j], (uint64_t)pp[j] | level_bits);
 }
 
 // Appends the lanes' flagged vertices at useg[pos ...] (ascending) or, with
@@ -518,7 +530,7 @@ __device__ inline uint32_t bu_append(const uint32_t (&v)[kBuIlp], const bool (&f
   for (int j = 0; j < kBuIlp; ++j) {
     const uint64_t m = __ballot(flag[j]);
     const uint32_t r = added + (uint32_t)__popcll(m & below);
-    if (flag[j]) useg[down ? pos - 1 - r : pos + r] = v[j];
+    if (flag[j]) __builtin_nontemporal_store(v[j], useg + (down ? pos - 1 - r : pos + r));
     added += (uint32_t)__popcll(m);
   }
   return added;
@@ -571,7 +583,7 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
 #pragma unroll
     for (int j = 0; j < kBuIlp; ++j) {
       okn[j] = !((readlane64(myword, j) >> lane) & 1ull);
-      fn[j] = okn[j] ? first[vbase + (uint32_t)(j * 64 + lane)] : make_uint2(0u, 0u);
+      fn[j] = okn[j] ? get_pair(first + vbase + (uint32_t)(j * 64 + lane)) : make_uint2(0u, 0u);
     }
     for (int j0 = 0; j0 < kWave; j0 += kBuIlp) {
       uint32_t v[kBuIlp];
@@ -587,14 +599,14 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
 #pragma unroll
         for (int j = 0; j < kBuIlp; ++j) {
           okn[j] = !((readlane64(myword, j0 + kBuIlp + j) >> lane) & 1ull);
-          fn[j] = okn[j] ? first[v[j] + kBuIlp * 64] : make_uint2(0u, 0u);
+          fn[j] = okn[j] ? get_pair(first + v[j] + kBuIlp * 64) : make_uint2(0u, 0u);
         }
       }
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(f[j].x);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j)
-        if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)f[j].y | level_bits);
+        if (hit[j]) put_pred(pred_new + v[j], (uint64_t)f[j].y | level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         const uint64_t hm = __ballot(hit[j]);
@@ -664,7 +676,7 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
           miss = !hit && dk >= dk1;
         }
         if (hit) {
-          pred_new[dv] = (int64_t)((uint64_t)n2o[par] | level_bits);
+          put_pred(pred_new + dv, (uint64_t)n2o[par] | level_bits);
           atomicOr(&fw[wave][(dv - vbase) >> 6], 1ull << (dv & 63));
         }
         const uint64_t mm = __ballot(miss);
@@ -729,7 +741,7 @@ __global__ __launch_bounds__(kT) void k_bu_long(
     }
     if (lane == 0) {
       if (found) {
-        pred_new[v] = (int64_t)((uint64_t)n2o[par] | level_bits);
+        put_pred(pred_new + v, (uint64_t)n2o[par] | level_bits);
         atomicOr(vis_new + (v >> 6), 1ull << (v & 63));
         ++nfound;
       } else {
@@ -784,7 +796,7 @@ __global__ __launch_bounds__(kT) void k_bu_list(
     }
 #pragma unroll
     for (int j = 0; j < kBuIlp; ++j)
-      fn[j] = (uint32_t)(j * kWave + lane) < n ? first[vn[j]] : make_uint2(0u, 0u);
+      fn[j] = (uint32_t)(j * kWave + lane) < n ? get_pair(first + vn[j]) : make_uint2(0u, 0u);
     for (uint32_t e0 = 0; e0 < n; e0 += kBuBatch) {
       uint32_t v[kBuIlp];
       uint2 f[kBuIlp];
@@ -803,14 +815,14 @@ __global__ __launch_bounds__(kT) void k_bu_list(
         }
 #pragma unroll
         for (int j = 0; j < kBuIlp; ++j)
-          fn[j] = e0 + kBuBatch + (uint32_t)(j * kWave + lane) < n ? first[vn[j]]
+          fn[j] = e0 + kBuBatch + (uint32_t)(j * kWave + lane) < n ? get_pair(first + vn[j])
                                                                    : make_uint2(0u, 0u);
       }
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) hit[j] = ok[j] && visited(f[j].x);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j)
-        if (hit[j]) pred_new[v[j]] = (int64_t)((uint64_t)f[j].y | level_bits);
+        if (hit[j]) put_pred(pred_new + v[j], (uint64_t)f[j].y | level_bits);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) scan[j] = ok[j] && !hit[j] && multi_bit(multi, v[j]);
       bool lng[kBuIlp];
@@ -924,7 +936,7 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
       if (p != kNoParent) e = (int64_t)((uint64_t)p | (0xFFFFull << 48));
     }
     if (o == root) e = root;
-    pred[o] = e;
+    __builtin_nontemporal_store(e, pred + o);
   }
 }
 
