@@ -940,6 +940,26 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
   }
 }
 
+// pred entries of the vertices reached after the snapshot the early gather
+// used (cur & ~snap): one thread per bitmap word, entries scattered by
+// original ID.
+__global__ __launch_bounds__(kT) void k_pred_fixup(const uint64_t* __restrict__ cur,
+                                                  const uint64_t* __restrict__ snap,
+                                                  int64_t words,
+                                                  const uint32_t* __restrict__ n2o,
+                                                  const int64_t* __restrict__ pred_new,
+                                                  int64_t* __restrict__ pred) {
+  const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (w >= words) return;
+  uint64_t bits = cur[w] & ~snap[w];
+  while (bits) {
+    const int b = __ffsll((long long)bits) - 1;
+    bits &= bits - 1;
+    const uint32_t v = (uint32_t)(w * 64 + b);
+    pred[n2o[v]] = pred_new[v];
+  }
+}
+
 // The walked path from the root to the core, with real depths.
 constexpr int kMaxPathArgs = 32;
 struct PathArgs {
@@ -1007,6 +1027,10 @@ void bfs_init(g500_bfs* b) {
   G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
+  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_snap, hipEventDisableTiming));
+  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
+  G500_CHECK_HIP(hipStreamCreate(&b->gstream));
+  b->vis_snap.alloc((size_t)b->words);
   ensure_host_orig2new(g);
   if (const char* ea = getenv("G500_EXEC_ALPHA")) b->exec_alpha = atof(ea);
   if (g->corebfs && !g->core_reset_done) {
@@ -1048,6 +1072,18 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
 
 // One bottom-up level: vis -> vis_alt, then the two swap.  `lists` walks
 // the unvisited lists the previous bottom-up level left, else the bitmap.
+// pred by original ID from the entries of the vertices set in `vis` (a
+// smaller grid beside the last top-down levels measured slower: the random
+// reads need every wave the chip holds).
+void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t st) {
+  g500_graph* g = b->g;
+  hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
+                     dim3(kT), 0, st, g->orig2new.get(), vis, b->pred_new.get(), g->n, g->n_c,
+                     root, g->corebfs ? g->tree_parent.get() : nullptr,
+                     g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
+  G500_CHECK_LAUNCH();
+}
+
 // The sweep lists its long rows in nq, which no bottom-up level uses (a
 // top-down level run as a sweep refills it from the bitmap afterwards).
 void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_bits) {
@@ -1121,6 +1157,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->trace.clear();
   b->stats = g500_run_stats{};
   b->have_result = false;
+  // a gather left running by a run that failed must not overlap this one
+  G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1);
@@ -1142,6 +1180,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   LevelSync* ls = b->lsync.get();
   unsigned long long rep[4];
   bool unv_ok = false;  // unvisited lists match the visited bitmap
+  bool early_gather = false;  // pred gather already running on gstream
 
   for (;;) {
     at[G500_AT_LEVEL] = ++level;
@@ -1239,7 +1278,18 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       qlen = nq_size;
       est_edges = nq_edges;
     } else if (next_td) {
-      // switching to top-down: frontier = bits new in this level
+      // switching to top-down: frontier = bits new in this level.  The
+      // remaining top-down levels are short and latency-bound, so the pred
+      // gather of every vertex reached so far starts now on the second
+      // stream, against a snapshot of the bitmap; the vertices the last
+      // levels reach are patched in at the end (k_pred_fixup)
+      G500_CHECK_HIP(hipMemcpyAsync(b->vis_snap.get(), b->vis.get(), words * sizeof(uint64_t),
+                                    hipMemcpyDeviceToDevice, s));
+      G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
+      G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
+      launch_gather(b, root, b->vis_snap.get(), b->gstream);
+      G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
+      early_gather = true;
       const unsigned long long cseq = ++b->seq;
       hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
                          0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
@@ -1254,11 +1304,15 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     }
     prev_nq = nq_size;
   }
-  hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
-                     dim3(kT), 0, s, g->orig2new.get(), b->vis.get(), b->pred_new.get(),
-                     g->n, n_c, root, g->corebfs ? g->tree_parent.get() : nullptr,
-                     g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
-  G500_CHECK_LAUNCH();
+  if (early_gather) {
+    G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
+    hipLaunchKernelGGL(k_pred_fixup, dim3((unsigned)ceil_div(words, kT)), dim3(kT), 0, s,
+                       b->vis.get(), b->vis_snap.get(), words, g->new2orig.get(),
+                       b->pred_new.get(), b->pred.get());
+    G500_CHECK_LAUNCH();
+  } else {
+    launch_gather(b, root, b->vis.get(), s);
+  }
   for (size_t i = 0; i < path.size(); i += kMaxPathArgs) {
     PathArgs pa;
     pa.count = (int)std::min<size_t>(kMaxPathArgs, path.size() - i);
@@ -1309,6 +1363,9 @@ g500_bfs::~g500_bfs() {
   if (h_report) (void)hipHostFree(h_report);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
+  if (ev_snap) (void)hipEventDestroy(ev_snap);
+  if (ev_gather) (void)hipEventDestroy(ev_gather);
+  if (gstream) (void)hipStreamDestroy(gstream);
   for (auto e : level_events) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }

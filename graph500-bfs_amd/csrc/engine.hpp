@@ -46,6 +46,11 @@ struct g500_bfs {
   g500::DevBuf<uint8_t> pred_valid;         // n
 
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  // the pred gather of the vertices reached by the last bottom-up level runs
+  // on gstream during the remaining top-down levels, against vis_snap
+  hipStream_t gstream = nullptr;
+  hipEvent_t ev_snap = nullptr, ev_gather = nullptr;
+  g500::DevBuf<uint64_t> vis_snap;
   std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
   bool timing = false;
   double exec_alpha = 15.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
