@@ -123,9 +123,12 @@ __device__ inline void finish_level(LevelSync* ls, HostReport* hr,
 
 // Bits of vertices >= n_c are pre-set so they are never searched
 // (the reference pre-visits vertices without edges, mpi/bfs.hpp:330-334).
+// Per-root init in one launch: the bitmap (plus the root's bit) and, from
+// thread 0, the root's pred entry and the one-vertex frontier.
 __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
                                                 int64_t words, int64_t n_c,
-                                                int64_t root_new) {
+                                                int64_t root_new, int64_t* pred_new,
+                                                int64_t entry, uint32_t* q) {
   for (int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x; w < words;
        w += (int64_t)gridDim.x * kT) {
     const int64_t lo = w * 64;
@@ -135,11 +138,7 @@ __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
     if (root_new >= lo && root_new < lo + 64) word |= 1ull << (root_new - lo);
     vis[w] = word;
   }
-}
-
-__global__ void k_bfs_root(int64_t* pred_new, int64_t entry, uint32_t* q,
-                           int64_t root_new, int64_t n_c) {
-  if (root_new >= 0 && root_new < n_c) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && root_new >= 0 && root_new < n_c) {
     pred_new[root_new] = entry;
     q[0] = (uint32_t)root_new;
   }
@@ -545,7 +544,7 @@ __device__ inline bool multi_bit(const uint64_t* multi, uint32_t v) {
 // k_bu_long, which gives each one a whole wave (one such row would otherwise
 // hold its wave far longer than the rest).  k_bu_long closes the level.
 constexpr int64_t kBuLongRow = 256;
-constexpr int kBuStep = 8;  // row entries a lane probes per step
+constexpr int kBuStep = 8;  // row entries a lane probes per step (two uint4 loads)
 
 __global__ __launch_bounds__(kT) void k_bu_sweep(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
@@ -615,8 +614,25 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
         fin[j] = ok[j] && !hit[j] && !more;
         defer[j] = ok[j] && !hit[j] && more;
       }
-      wp += bu_append(v, fin, useg, wp, false);
-      dn += bu_append(v, defer, useg, kChunkVerts - dn, true);
+      // one store per slot: final misses to the front, deferred rows to the back
+      {
+        const uint64_t below = lanemask_lt();
+        uint32_t addf = 0, addd = 0;
+#pragma unroll
+        for (int j = 0; j < kBuIlp; ++j) {
+          const uint64_t mf = __ballot(fin[j]), md = __ballot(defer[j]);
+          if (fin[j] || defer[j]) {
+            const uint32_t pos =
+                fin[j] ? wp + addf + (uint32_t)__popcll(mf & below)
+                       : kChunkVerts - dn - 1 - (addd + (uint32_t)__popcll(md & below));
+            __builtin_nontemporal_store(v[j], useg + pos);
+          }
+          addf += (uint32_t)__popcll(mf);
+          addd += (uint32_t)__popcll(md);
+        }
+        wp += addf;
+        dn += addd;
+      }
     }
     // deferred rows: a lane takes the next one (lowest segment position
     // first) whenever it has none, and scans kBuStep entries per step, so a
@@ -631,7 +647,7 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
       uint32_t taken = 0;  // wave-uniform
       bool have = false;
       uint32_t dv = 0;
-      int64_t dk = 0, dk1 = 0;
+      int64_t dlo = 0, dk = 0, dk1 = 0;  // row part [dlo, dk1); next step at dk (16 B aligned)
       const uint64_t below = lanemask_lt();
       for (;;) {
         const uint64_t idle = __ballot(!have);
@@ -641,10 +657,11 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
         bool lng = false;
         if (!have && idx < dn) {
           dv = useg[dbase + idx];
-          dk = row_off[dv] + 1;
+          dlo = row_off[dv] + 1;
           dk1 = row_off[dv + 1];
+          dk = dlo & ~(int64_t)3;
           have = true;
-          lng = dk1 - dk > kBuLongRow;
+          lng = dk1 - dlo > kBuLongRow;
         }
         if (__ballot(have) == 0) break;
         const uint64_t lm = __ballot(lng);
@@ -660,12 +677,16 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
         bool hit = false, miss = false;
         uint32_t par = 0;
         if (have) {
-          uint32_t u[kBuStep];
-#pragma unroll
-          for (int t = 0; t < kBuStep; ++t) u[t] = dk + t < dk1 ? col[dk + t] : 0u;
+          // two aligned 16-byte loads per step (col carries a few padding
+          // entries past m); entries outside [dlo, dk1) are not probed
+          const uint4* c4 = reinterpret_cast<const uint4*>(col) + (dk >> 2);
+          const uint4 a = c4[0];
+          const uint4 b4 = dk + 4 < dk1 ? c4[1] : make_uint4(0u, 0u, 0u, 0u);
+          const uint32_t u[kBuStep] = {a.x, a.y, a.z, a.w, b4.x, b4.y, b4.z, b4.w};
           bool h[kBuStep];
 #pragma unroll
-          for (int t = 0; t < kBuStep; ++t) h[t] = dk + t < dk1 && visited(u[t]);
+          for (int t = 0; t < kBuStep; ++t)
+            h[t] = dk + t >= dlo && dk + t < dk1 && visited(u[t]);
 #pragma unroll
           for (int t = kBuStep - 1; t >= 0; --t)
             if (h[t]) {
@@ -1161,11 +1182,9 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
-                     b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1);
-  G500_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, b->pred_new.get(),
-                     (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get(),
-                     root_in ? root_new : (int64_t)-1, n_c);
+                     b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1,
+                     b->pred_new.get(),
+                     (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get());
   G500_CHECK_LAUNCH();
 
   int64_t qlen = root_in ? 1 : 0;

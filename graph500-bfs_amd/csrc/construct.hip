@@ -330,7 +330,9 @@ void build_csr(g500_graph* g, const uint2* edges, int64_t ne, bool symmetric,
                                   hipMemcpyDeviceToHost, s));
     G500_CHECK_HIP(hipStreamSynchronize(s));
     g->m = (int64_t)m_total;
-    g->col.alloc((size_t)g->m);
+    // padding: bottom-up scans read aligned 16-byte groups, which may run a
+    // few entries past the last row
+    g->col.alloc((size_t)g->m + kColPad);
     g->row_off.alloc((size_t)g->n_c + 1);
     hipLaunchKernelGGL(k_dedup_write, dim3((unsigned)tiles), dim3(kT), 0, s,
                        sorted, ms, lg, toff.get(), g->col.get(), g->row_off.get());

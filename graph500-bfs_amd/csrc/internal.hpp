@@ -72,6 +72,9 @@ void ensure_host_orig2new(g500_graph* g);
 
 // corebfs.hip
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;
+// entries allocated past the end of col (aligned 16-byte loads of a row's
+// last group may read up to three of them; their values are never used)
+constexpr int64_t kColPad = 16;
 void corebfs_prune(g500_graph* g);
 // reachability reset of the forest after a BFS from hd_root over the core;
 // `visited` is that BFS's visited bitmap over core new IDs
