@@ -884,7 +884,7 @@ __global__ __launch_bounds__(kT) void k_bitmap_to_queue(
     const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
     int64_t words, const uint32_t* __restrict__ deg_new,
     uint32_t* __restrict__ out, LevelSync* ls, HostReport* hr,
-    unsigned long long seq) {
+    unsigned long long seq, uint64_t* __restrict__ snap) {
   __shared__ uint32_t stage[kQueueWords * 64];
   __shared__ uint64_t wsum[kWavesPerBlock];
   __shared__ unsigned long long base;
@@ -899,7 +899,12 @@ __global__ __launch_bounds__(kT) void k_bitmap_to_queue(
     return w < words ? (uint32_t)(((cur[w] & ~prev[w]) >> q) & 0xFFFFu) : 0u;
   };
   uint64_t mine = 0;
-  for (int64_t g = g0; g < g1; ++g) mine += (uint64_t)__popc(quarter(g));
+  for (int64_t g = g0; g < g1; ++g) {
+    mine += (uint64_t)__popc(quarter(g));
+    // optional copy of cur (the pred gather's snapshot)
+    const int64_t w = g * kQueueWords + t / 4;
+    if (snap && (t & 3) == 0 && w < words) snap[w] = cur[w];
+  }
   const uint64_t all = block_sum(mine, wsum);
   if (t == 0) base = all ? atomicAdd(&ls->cnt[0], (unsigned long long)all) : 0;
   __syncthreads();
@@ -1229,7 +1234,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       seq = ++b->seq;
       hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
                          0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
-                         b->nq.get(), ls, b->d_report, seq);
+                         b->nq.get(), ls, b->d_report, seq, (uint64_t*)nullptr);
       G500_CHECK_LAUNCH();
     } else if (td) {
       // small frontiers: the expansion stages them itself
@@ -1299,21 +1304,20 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     } else if (next_td) {
       // switching to top-down: frontier = bits new in this level.  The
       // remaining top-down levels are short and latency-bound, so the pred
-      // gather of every vertex reached so far starts now on the second
-      // stream, against a snapshot of the bitmap; the vertices the last
-      // levels reach are patched in at the end (k_pred_fixup)
-      G500_CHECK_HIP(hipMemcpyAsync(b->vis_snap.get(), b->vis.get(), words * sizeof(uint64_t),
-                                    hipMemcpyDeviceToDevice, s));
+      // gather of every vertex reached so far starts on the second stream
+      // once the conversion (which also snapshots the bitmap) is done; the
+      // vertices the last levels reach are patched in at the end
+      // (k_pred_fixup)
+      const unsigned long long cseq = ++b->seq;
+      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
+                         0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
+                         b->q.get(), ls, b->d_report, cseq, b->vis_snap.get());
+      G500_CHECK_LAUNCH();
       G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
       G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
       launch_gather(b, root, b->vis_snap.get(), b->gstream);
       G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
       early_gather = true;
-      const unsigned long long cseq = ++b->seq;
-      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
-                         0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
-                         b->q.get(), ls, b->d_report, cseq);
-      G500_CHECK_LAUNCH();
       wait_report(b, cseq, rep);
       qlen = (int64_t)rep[0];
       est_edges = (int64_t)rep[1];
