@@ -966,6 +966,39 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
   }
 }
 
+// Without CoreBFS, a vertex without edges is never reached and its pred
+// entry is always -1: the engine fills pred once and the gather walks only
+// core_o, the original IDs of the vertices with edges in ascending order.
+__global__ __launch_bounds__(kT) void k_pred_gather_core(const uint32_t* __restrict__ core_o,
+                                                        int64_t n_core,
+                                                        const uint32_t* __restrict__ o2n,
+                                                        const uint64_t* __restrict__ vis,
+                                                        const int64_t* __restrict__ pred_new,
+                                                        int64_t root,
+                                                        int64_t* __restrict__ pred) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n_core;
+       i += (int64_t)gridDim.x * kT) {
+    const uint32_t o = core_o[i];
+    const uint32_t v = o2n[o];
+    int64_t e = ((vis[v >> 6] >> (v & 63)) & 1ull) ? pred_new[v] : -1;
+    if ((int64_t)o == root) e = root;
+    __builtin_nontemporal_store(e, pred + o);
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_core_flags(const uint32_t* __restrict__ o2n, int64_t n,
+                                                  int64_t n_c, uint32_t* __restrict__ flag) {
+  const int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (o < n) flag[o] = (int64_t)o2n[o] < n_c ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kT) void k_core_list(const uint32_t* __restrict__ flag,
+                                                 const uint64_t* __restrict__ off, int64_t n,
+                                                 uint32_t* __restrict__ core_o) {
+  const int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (o < n && flag[o]) core_o[off[o]] = (uint32_t)o;
+}
+
 // pred entries of the vertices reached after the snapshot the early gather
 // used (cur & ~snap): one thread per bitmap word, entries scattered by
 // original ID.
@@ -1057,6 +1090,33 @@ void bfs_init(g500_bfs* b) {
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
   G500_CHECK_HIP(hipStreamCreate(&b->gstream));
   b->vis_snap.alloc((size_t)b->words);
+  if (!g->corebfs && g->n_c > 0) {
+    // the core's original IDs in ascending order; pred starts at -1, which
+    // the vertices without edges keep
+    hipStream_t s = b->stream;
+    DevBuf<uint32_t> flag;
+    DevBuf<uint64_t> off;
+    flag.alloc((size_t)g->n);
+    off.alloc((size_t)g->n + 1);
+    hipLaunchKernelGGL(k_core_flags, dim3((unsigned)ceil_div(g->n, kT)), dim3(kT), 0, s,
+                       g->orig2new.get(), g->n, g->n_c, flag.get());
+    G500_CHECK_LAUNCH();
+    ScanWorkspace ws;
+    exclusive_scan<uint32_t>(flag.get(), off.get(), g->n, ws, s);
+    b->core_o.alloc((size_t)g->n_c);
+    hipLaunchKernelGGL(k_core_list, dim3((unsigned)ceil_div(g->n, kT)), dim3(kT), 0, s,
+                       flag.get(), off.get(), g->n, b->core_o.get());
+    G500_CHECK_LAUNCH();
+    uint64_t total = 0;
+    G500_CHECK_HIP(hipMemcpyAsync(&total, off.get() + g->n, sizeof(total),
+                                  hipMemcpyDeviceToHost, s));
+    G500_CHECK_HIP(hipStreamSynchronize(s));
+    if ((int64_t)total != g->n_c)
+      fail(G500_ERR_STATE, "core list holds %llu vertices, expected %lld",
+           (unsigned long long)total, (long long)g->n_c);
+    b->n_core = g->n_c;
+    b->pred_fill_needed = true;
+  }
   ensure_host_orig2new(g);
   if (const char* ea = getenv("G500_EXEC_ALPHA")) b->exec_alpha = atof(ea);
   if (g->corebfs && !g->core_reset_done) {
@@ -1103,10 +1163,16 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
 // reads need every wave the chip holds).
 void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t st) {
   g500_graph* g = b->g;
-  hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
-                     dim3(kT), 0, st, g->orig2new.get(), vis, b->pred_new.get(), g->n, g->n_c,
-                     root, g->corebfs ? g->tree_parent.get() : nullptr,
-                     g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
+  if (b->n_core > 0)
+    hipLaunchKernelGGL(k_pred_gather_core,
+                       dim3((unsigned)std::min<int64_t>(ceil_div(b->n_core, kT), 65536)),
+                       dim3(kT), 0, st, b->core_o.get(), b->n_core, g->orig2new.get(), vis,
+                       b->pred_new.get(), root, b->pred.get());
+  else
+    hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
+                       dim3(kT), 0, st, g->orig2new.get(), vis, b->pred_new.get(), g->n, g->n_c,
+                       root, g->corebfs ? g->tree_parent.get() : nullptr,
+                       g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
   G500_CHECK_LAUNCH();
 }
 
@@ -1185,6 +1251,13 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->have_result = false;
   // a gather left running by a run that failed must not overlap this one
   G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
+  if (b->n_core > 0 && b->pred_fill_needed) {
+    // entries of vertices without edges are -1 and stay so; the fill is
+    // redone only after the buffer was overwritten (validation of a host pred)
+    G500_CHECK_HIP(hipMemsetAsync(b->pred.get(), 0xFF, (size_t)g->n * sizeof(int64_t), s));
+    b->pred_fill_needed = false;
+    b->iso_root = -1;
+  }
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1,
@@ -1335,6 +1408,16 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     G500_CHECK_LAUNCH();
   } else {
     launch_gather(b, root, b->vis.get(), s);
+  }
+  if (b->n_core > 0) {
+    // the gather skips vertices without edges: a root among them gets its
+    // own entry here, and the last such root's entry goes back to -1
+    if (b->iso_root >= 0 && b->iso_root != root) path.emplace_back((uint32_t)b->iso_root, -1);
+    b->iso_root = -1;
+    if (!root_in) {
+      path.emplace_back((uint32_t)root, root);
+      b->iso_root = root;
+    }
   }
   for (size_t i = 0; i < path.size(); i += kMaxPathArgs) {
     PathArgs pa;

@@ -51,6 +51,12 @@ struct g500_bfs {
   hipStream_t gstream = nullptr;
   hipEvent_t ev_snap = nullptr, ev_gather = nullptr;
   g500::DevBuf<uint64_t> vis_snap;
+  // without CoreBFS: original IDs of the vertices with edges, ascending (the
+  // gather's domain; every other pred entry stays -1)
+  g500::DevBuf<uint32_t> core_o;
+  int64_t n_core = 0;
+  int64_t iso_root = -1;  // a root without edges whose entry the last run wrote
+  bool pred_fill_needed = false;  // pred must be reset to -1 before the next run
   std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
   bool timing = false;
   double exec_alpha = 15.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up

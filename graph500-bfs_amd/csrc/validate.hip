@@ -159,6 +159,7 @@ void bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
   if (host_pred) {
     G500_CHECK_HIP(hipMemcpyAsync(b->pred.get(), host_pred, nlocal * sizeof(int64_t),
                                   hipMemcpyHostToDevice, s));
+    b->pred_fill_needed = true;  // the next BFS run restores the -1 entries
   } else if (!b->have_result) {
     fail(G500_ERR_STATE, "validate: no BFS result on the device");
   }

@@ -205,6 +205,34 @@ def test_validator_agrees_on_planted_mistakes(G):
         assert first(e_d) == first(e_o), (e_d, e_o)
     ok, te, _ = bfs.validate(r, host_pred=pred)
     assert ok and te == 65535
+    # a run after uploads of other parent arrays writes every entry afresh
+    bfs.validate(r, host_pred=np.full_like(pred, 5))
+    r2 = int(g.find_roots(2)[1])
+    bfs.run(r2)
+    ok, te, errs = bfs.validate(r2)
+    assert ok and te == 65535, errs
+    assert og.validate(r2, bfs.pred())[0]
+
+
+def test_roots_without_edges(G):
+    """Roots that have no edges reach only themselves; the next run, from
+    any root, leaves no trace of them in the parent array."""
+    print("\n[isolated roots]", flush=True)
+    v0, v1 = _small_cases()["two_components"]
+    v0 = np.asarray(v0, np.int64)
+    v1 = np.asarray(v1, np.int64)
+    g = G.Graph.from_edges(v0, v1).construct()
+    og = O.Graph(v0, v1)
+    bfs = G.Bfs(g)
+    for r in (5, 7, 7, 0, 6, 11, 2):
+        bfs.run(r)
+        assert np.array_equal(bfs.levels(), _oracle_levels(og, r)), r
+        pred = bfs.pred()
+        ok, _, errs = og.validate(r, pred)
+        assert ok, (r, errs)
+        assert bfs.validate(r)[0], r
+        lone = [v for v in range(og.info.n) if not og.has_edge(v)]
+        assert all(pred[v] == (r if v == r else -1) for v in lone), (r, pred.tolist())
 
 
 @pytest.mark.parametrize("exec_alpha", [0.0, 1e9, None])
