@@ -26,15 +26,17 @@
 //   k_bu_list     later bottom-up levels: the same probes over those lists,
 //                 compacted in place as vertices are found.  New bits are
 //                 written once per bitmap word either way.
-//   k_bitmap_to_queue   bitmap -> queue at the bottom-up -> top-down switch.
+//   k_queue_count, k_queue_write   bitmap -> queue at the bottom-up ->
+//                 top-down switch and after top-down levels run as sweeps.
 //   k_pred_gather after the last level: pred[o] for every original ID o,
 //                 gathered from the entries the levels wrote by new ID
 //                 (levels write densely by new ID; one coalesced pass then
 //                 produces pred in original-ID order).
 //
-// Level hand-off: the kernel that closes a level accumulates its counters in
-// device memory; its last workgroup (atomic ticket) posts the totals and a
-// sequence number to pinned host memory and clears the counters.  The host
+// Level hand-off: every block of a level's kernels writes its totals to a
+// per-block slot; a one-block report kernel queued behind them sums the
+// slots, posts the totals and a sequence number to pinned host memory and
+// clears the level counters.  The host
 // spins on the sequence number, decides the next direction exactly as the
 // reference does, and launches the next level -- no copies, memsets or
 // stream synchronisations per level.
@@ -103,22 +105,44 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t* wsum,
   return x - v + off;
 }
 
-// Called by thread 0 of every block once its totals are in ls->cnt.
-__device__ inline void finish_level(LevelSync* ls, HostReport* hr,
-                                    unsigned long long seq) {
-  __threadfence();
-  const unsigned t = atomicAdd(&ls->done, 1u);
-  if (t != gridDim.x - 1) return;
-  __threadfence();
-  unsigned long long v[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = atomicExch(&ls->cnt[i], 0ull);
-  atomicExch(&ls->done, 0u);
+// Level totals and their sequence number to the host's pinned report.
+__device__ inline void post_report(HostReport* hr, const unsigned long long v[4],
+                                   unsigned long long seq) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     __hip_atomic_store(&hr->v[i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __threadfence_system();
   __hip_atomic_store(&hr->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// One block, launched after the level's kernels: posts the level totals --
+// the blocks' partial sums (counts in cpart, degree sums in epart) plus what
+// queue reservations took from ls->cnt -- and clears ls->cnt for the next
+// level.  The blocks of a level kernel finish together, so summing their
+// totals here replaces same-address atomics, which serialize.  hr may be
+// null (totals not needed, counters still cleared).
+__global__ __launch_bounds__(kT) void k_level_report(const uint64_t* __restrict__ cpart,
+                                                    int nc,
+                                                    const uint64_t* __restrict__ epart,
+                                                    int ne, LevelSync* ls, HostReport* hr,
+                                                    unsigned long long seq) {
+  __shared__ uint64_t red[kWavesPerBlock];
+  uint64_t c = 0, d = 0;
+  for (int i = threadIdx.x; i < nc; i += kT) c += cpart[i];
+  for (int i = threadIdx.x; i < ne; i += kT) d += epart[i];
+  c = block_sum(c, red);
+  d = block_sum(d, red);
+  if (threadIdx.x == 0) {
+    unsigned long long v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = __hip_atomic_load(&ls->cnt[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ls->cnt[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    v[0] += c;
+    v[1] += d;
+    if (hr) post_report(hr, v, seq);
+  }
 }
 
 // Bits of vertices >= n_c are pre-set so they are never searched
@@ -272,7 +296,7 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     const uint32_t* __restrict__ col, const uint32_t* __restrict__ n2o,
     const uint32_t* __restrict__ deg_new, unsigned long long* vis,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ nq, LevelSync* ls,
-    HostReport* hr, unsigned long long seq, uint64_t level_bits) {
+    uint64_t* __restrict__ epart, uint64_t level_bits) {
   // the tile's owners, staged once per tile: prefix sum, col offset of edge
   // 0 of the owner's range (row_off[u] - dsum[i]) and original ID.  On heavy
   // levels a tile has one or two owners (hubs) shared by every block at
@@ -419,10 +443,7 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     __syncthreads();
   }
   const uint64_t s = block_sum(my_edges, red);
-  if (t == 0) {
-    if (s) atomicAdd(&ls->cnt[1], (unsigned long long)s);
-    finish_level(ls, hr, seq);
-  }
+  if (t == 0) epart[blockIdx.x] = s;
 }
 
 // Bottom-up levels.  A wave owns a chunk of 64 bitmap words (4096 vertices)
@@ -553,7 +574,7 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
     uint32_t* __restrict__ unv_cnt, uint32_t* __restrict__ longv, LevelSync* ls,
-    uint64_t level_bits) {
+    uint64_t* __restrict__ cpart, uint64_t level_bits) {
   __shared__ unsigned long long fw[kWavesPerBlock][kWave];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ uint64_t hub[kHubWords];
@@ -714,7 +735,7 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
     wave_sync();
   }
   const uint64_t s = block_sum(nfound, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&ls->cnt[0], (unsigned long long)s);
+  if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
 // The rows k_bu_sweep listed, a wave each: 256 entries per step, first
@@ -725,8 +746,8 @@ __global__ __launch_bounds__(kT) void k_bu_long(
     const int64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
     const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
     uint32_t* __restrict__ unv, uint32_t* unv_cnt,
-    const uint32_t* __restrict__ longv, LevelSync* ls, HostReport* hr,
-    unsigned long long seq, uint64_t level_bits) {
+    const uint32_t* __restrict__ longv, const LevelSync* ls,
+    uint64_t* __restrict__ cpart, uint64_t level_bits) {
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
@@ -773,10 +794,7 @@ __global__ __launch_bounds__(kT) void k_bu_long(
     }
   }
   const uint64_t s = block_sum(nfound, red);
-  if (threadIdx.x == 0) {
-    if (s) atomicAdd(&ls->cnt[0], (unsigned long long)s);
-    finish_level(ls, hr, seq);
-  }
+  if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(kT) void k_bu_list(
@@ -785,8 +803,7 @@ __global__ __launch_bounds__(kT) void k_bu_list(
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
-    uint32_t* __restrict__ unv_cnt, LevelSync* ls, HostReport* hr,
-    unsigned long long seq, uint64_t level_bits) {
+    uint32_t* __restrict__ unv_cnt, uint64_t* __restrict__ cpart, uint64_t level_bits) {
   __shared__ unsigned long long fw[kWavesPerBlock][kWave];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ uint64_t hub[kHubWords];
@@ -864,79 +881,116 @@ __global__ __launch_bounds__(kT) void k_bu_list(
     wave_sync();
   }
   const uint64_t s = block_sum(nfound, red);
-  if (threadIdx.x == 0) {
-    if (s) atomicAdd(&ls->cnt[0], (unsigned long long)s);
-    finish_level(ls, hr, seq);
-  }
+  if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
 // queue <- bits set in cur but not in prev; reports (count, degree sum).
-// Block b owns a contiguous range of 64-word groups: one pass counts its new
-// vertices and reserves their queue range with a single atomic (same-address
-// atomics serialize, so the grid stays small), the second takes the groups
-// one by one: thread t compacts quarter t % 4 of word t / 4 into LDS (at most
-// 4096 vertices per group), then the block writes its slice coalesced and
-// loads the degrees in parallel.
-constexpr int kQueueWords = kT / 4;
+// Two launches over one partition of the words: block b owns a contiguous
+// range and each of its waves a contiguous quarter of it.  k_queue_count
+// writes every wave's count (and, when asked, the snapshot copy of cur for
+// the pred gather); k_queue_write takes its offset from the counts of the
+// waves before it, so no block waits on a shared reservation, and each wave
+// compacts its own words with ballots: lanes store the queue and load the
+// degrees coalesced, with up to four words' degree loads in flight.
 constexpr int kQueueGrid = 1024;
 
-__global__ __launch_bounds__(kT) void k_bitmap_to_queue(
-    const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
-    int64_t words, const uint32_t* __restrict__ deg_new,
-    uint32_t* __restrict__ out, LevelSync* ls, HostReport* hr,
-    unsigned long long seq, uint64_t* __restrict__ snap) {
-  __shared__ uint32_t stage[kQueueWords * 64];
-  __shared__ uint64_t wsum[kWavesPerBlock];
-  __shared__ unsigned long long base;
-  const int t = threadIdx.x;
-  const int q = (t & 3) * 16;
-  const int64_t groups = ceil_div(words, kQueueWords);
-  const int64_t per = ceil_div(groups, (int64_t)gridDim.x);
-  const int64_t g0 = (int64_t)blockIdx.x * per;
-  const int64_t g1 = min(g0 + per, groups);
-  auto quarter = [&](int64_t g) -> uint32_t {
-    const int64_t w = g * kQueueWords + t / 4;
-    return w < words ? (uint32_t)(((cur[w] & ~prev[w]) >> q) & 0xFFFFu) : 0u;
-  };
-  uint64_t mine = 0;
-  for (int64_t g = g0; g < g1; ++g) {
-    mine += (uint64_t)__popc(quarter(g));
-    // optional copy of cur (the pred gather's snapshot)
-    const int64_t w = g * kQueueWords + t / 4;
-    if (snap && (t & 3) == 0 && w < words) snap[w] = cur[w];
+__device__ inline void queue_wave_span(int64_t words, int64_t* a, int64_t* e) {
+  const int64_t per_block = ceil_div(words, (int64_t)gridDim.x);
+  const int64_t b0 = min((int64_t)blockIdx.x * per_block, words);
+  const int64_t b1 = min(b0 + per_block, words);
+  const int64_t per_wave = ceil_div(b1 - b0, (int64_t)kWavesPerBlock);
+  *a = min(b0 + (int64_t)(threadIdx.x / kWave) * per_wave, b1);
+  *e = min(*a + per_wave, b1);
+}
+
+__global__ __launch_bounds__(kT) void k_queue_count(const uint64_t* __restrict__ cur,
+                                                   const uint64_t* __restrict__ prev,
+                                                   int64_t words, uint32_t* __restrict__ wcnt,
+                                                   uint64_t* __restrict__ snap) {
+  int64_t a, e;
+  queue_wave_span(words, &a, &e);
+  uint64_t c = 0;
+  for (int64_t w = a + lane_id(); w < e; w += kWave) {
+    const uint64_t x = cur[w];
+    c += (uint64_t)__popcll(x & ~prev[w]);
+    if (snap) snap[w] = x;
   }
-  const uint64_t all = block_sum(mine, wsum);
-  if (t == 0) base = all ? atomicAdd(&ls->cnt[0], (unsigned long long)all) : 0;
+  c = wave_sum_u64(c);
+  if (lane_id() == 0) wcnt[(int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave] = (uint32_t)c;
+}
+
+__global__ __launch_bounds__(kT) void k_queue_write(const uint64_t* __restrict__ cur,
+                                                   const uint64_t* __restrict__ prev,
+                                                   int64_t words,
+                                                   const uint32_t* __restrict__ deg_new,
+                                                   const uint32_t* __restrict__ wcnt,
+                                                   uint32_t* __restrict__ out,
+                                                   uint64_t* __restrict__ bedges) {
+  __shared__ uint64_t red[kWavesPerBlock];
+  const int t = threadIdx.x, lane = lane_id(), wave = t / kWave;
+  const int64_t first = (int64_t)blockIdx.x * kWavesPerBlock;
+  uint64_t before = 0;
+  for (int64_t i = t; i < first; i += kT) before += wcnt[i];
+  before = wave_sum_u64(before);
+  if (lane == 0) red[wave] = before;
   __syncthreads();
-  uint64_t off = base, edges = 0;
-  for (int64_t g = g0; g < g1; ++g) {
-    uint32_t bits = quarter(g);
-    uint64_t tot;
-    uint32_t pos = (uint32_t)block_excl_scan((uint64_t)__popc(bits), wsum, &tot);
-    const int64_t w = g * kQueueWords + t / 4;
-    while (bits) {
-      const int bit = __ffs(bits) - 1;
-      bits &= bits - 1;
-      stage[pos++] = (uint32_t)(w * 64 + q + bit);
-    }
-    __syncthreads();
-    const uint32_t n = (uint32_t)tot;
+  uint64_t off = 0;
 #pragma unroll
-    for (int r = 0; r < kQueueWords * 64 / kT; ++r) {
-      const uint32_t i = (uint32_t)(r * kT + t);
-      if (i < n) {
-        const uint32_t v = stage[i];
-        out[off + i] = v;
-        edges += deg_new[v];
+  for (int w = 0; w < kWavesPerBlock; ++w) off += red[w];
+  for (int w = 0; w < wave; ++w) off += wcnt[first + w];
+  int64_t a, e;
+  queue_wave_span(words, &a, &e);
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint64_t edges = 0;
+  for (int64_t g = a; g < e; g += kWave) {
+    const int64_t w = g + lane;
+    const uint64_t mine = w < e ? (cur[w] & ~prev[w]) : 0ull;
+    uint64_t nz = __ballot(mine != 0ull);
+    while (nz) {
+      uint32_t d[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        d[u] = 0;
+        if (nz) {
+          const int j = __ffsll((unsigned long long)nz) - 1;
+          nz &= nz - 1;
+          const uint64_t x = readlane64(mine, j);
+          if ((x >> lane) & 1ull) {
+            const uint32_t v = (uint32_t)((g + j) * 64 + lane);
+            out[off + (uint64_t)__popcll(x & below)] = v;
+            d[u] = deg_new[v];
+          }
+          off += (uint64_t)__popcll(x);
+        }
       }
+      edges += (uint64_t)d[0] + d[1] + d[2] + d[3];
     }
-    off += tot;
-    __syncthreads();
   }
-  const uint64_t s = block_sum(edges, wsum);
-  if (t == 0) {
-    if (s) atomicAdd(&ls->cnt[1], (unsigned long long)s);
-    finish_level(ls, hr, seq);
+  __syncthreads();  // red is reused
+  const uint64_t s = block_sum(edges, red);
+  if (t == 0) bedges[blockIdx.x] = s;
+}
+
+// One block: the conversion's totals (count, degree sum) to the host.  All
+// blocks of k_queue_write finish together, so their totals are summed here
+// rather than by atomics on one address, which serialize.
+__global__ __launch_bounds__(kT) void k_queue_report(const uint32_t* __restrict__ wcnt,
+                                                    const uint64_t* __restrict__ bedges,
+                                                    int blocks, LevelSync* ls, HostReport* hr,
+                                                    unsigned long long seq) {
+  __shared__ uint64_t red[kWavesPerBlock];
+  uint64_t c = 0, d = 0;
+  for (int i = threadIdx.x; i < blocks * kWavesPerBlock; i += kT) c += wcnt[i];
+  for (int i = threadIdx.x; i < blocks; i += kT) d += bedges[i];
+  c = block_sum(c, red);
+  d = block_sum(d, red);
+  if (threadIdx.x == 0) {
+    // a top-down level run as a sweep leaves the long-row count behind
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      __hip_atomic_store(&ls->cnt[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long v[4] = {c, d, 0ull, 0ull};
+    post_report(hr, v, seq);
   }
 }
 
@@ -1078,6 +1132,8 @@ void bfs_init(g500_bfs* b) {
     b->bu_grid[i] = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)std::max(cus, 1) * std::max(per_cu[i], 1),
                              ceil_div(chunks, kWavesPerBlock)));
+  b->lpart.alloc((size_t)std::max<int64_t>(
+      kMaxGrid, std::max<int64_t>(b->bu_grid[0] + b->bu_grid[2], b->bu_grid[1])));
   b->lsync.alloc(1);
   G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
   G500_CHECK_HIP(hipHostMalloc((void**)&b->h_report, sizeof(HostReport),
@@ -1090,6 +1146,8 @@ void bfs_init(g500_bfs* b) {
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
   G500_CHECK_HIP(hipStreamCreate(&b->gstream));
   b->vis_snap.alloc((size_t)b->words);
+  b->qcnt.alloc((size_t)kQueueGrid * kWavesPerBlock);
+  b->qedges.alloc((size_t)kQueueGrid);
   if (!g->corebfs && g->n_c > 0) {
     // the core's original IDs in ascending order; pred starts at -1, which
     // the vertices without edges keep
@@ -1178,30 +1236,56 @@ void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t s
 
 // The sweep lists its long rows in nq, which no bottom-up level uses (a
 // top-down level run as a sweep refills it from the bitmap afterwards).
-void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_bits) {
+void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_bits,
+               bool report) {
   g500_graph* g = b->g;
   hipStream_t s = b->stream;
+  int parts = 0;
   if (lists) {
     hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kT), 0, s, b->vis.get(),
                        b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
                        g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
-                       b->unv.get(), b->unv_cnt.get(), b->lsync.get(), b->d_report, seq,
-                       level_bits);
+                       b->unv.get(), b->unv_cnt.get(), b->lpart.get(), level_bits);
     G500_CHECK_LAUNCH();
+    parts = (int)b->bu_grid[1];
   } else {
     hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kT), 0, s, b->vis.get(),
                        b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
                        g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
-                       b->unv.get(), b->unv_cnt.get(), b->nq.get(), b->lsync.get(), level_bits);
+                       b->unv.get(), b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
+                       b->lpart.get(), level_bits);
     G500_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kT), 0, s, b->vis.get(),
                        (unsigned long long*)b->vis_alt.get(), b->words, g->row_off.get(),
                        g->col.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
-                       b->unv_cnt.get(), b->nq.get(), b->lsync.get(), b->d_report, seq,
-                       level_bits);
+                       b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
+                       b->lpart.get() + b->bu_grid[0], level_bits);
+    G500_CHECK_LAUNCH();
+    parts = (int)(b->bu_grid[0] + b->bu_grid[2]);
+  }
+  if (report) {
+    hipLaunchKernelGGL(k_level_report, dim3(1), dim3(kT), 0, s, b->lpart.get(), parts,
+                       (const uint64_t*)nullptr, 0, b->lsync.get(), b->d_report, seq);
     G500_CHECK_LAUNCH();
   }
   std::swap(b->vis.p, b->vis_alt.p);
+}
+
+// Frontier queue from the bits of the level just finished (vis minus
+// vis_alt, the bitmap before it); the level report carries its size and
+// degree sum.
+void launch_to_queue(g500_bfs* b, uint32_t* out, unsigned long long seq, uint64_t* snap) {
+  hipStream_t s = b->stream;
+  const dim3 grid((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(b->words, (int64_t)kT)));
+  hipLaunchKernelGGL(k_queue_count, grid, dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(),
+                     b->words, b->qcnt.get(), snap);
+  G500_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_queue_write, grid, dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(),
+                     b->words, b->g->deg_new.get(), b->qcnt.get(), out, b->qedges.get());
+  G500_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_queue_report, dim3(1), dim3(kT), 0, s, b->qcnt.get(), b->qedges.get(),
+                     (int)grid.x, b->lsync.get(), b->d_report, seq);
+  G500_CHECK_LAUNCH();
 }
 
 }  // namespace
@@ -1302,13 +1386,10 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
         !td || (b->exec_alpha > 0 && est_edges > 0 &&
                 (double)est_edges > (double)g->m / b->exec_alpha);
     if (td && exec_bu) {
-      launch_bu(b, false, seq, level_bits);
+      launch_bu(b, false, seq, level_bits, false);
       unv_ok = true;
       seq = ++b->seq;
-      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
-                         0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
-                         b->nq.get(), ls, b->d_report, seq, (uint64_t*)nullptr);
-      G500_CHECK_LAUNCH();
+      launch_to_queue(b, b->nq.get(), seq, nullptr);
     } else if (td) {
       // small frontiers: the expansion stages them itself
       if (qlen > kTdWin) {
@@ -1326,11 +1407,14 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                          b->q.get(), qlen, b->dsum.get(), b->tile_owner.get(), g->row_off.get(),
                          g->col.get(), g->new2orig.get(), g->deg_new.get(),
                          (unsigned long long*)b->vis.get(), b->pred_new.get(), b->nq.get(),
-                         ls, b->d_report, seq, level_bits);
+                         ls, b->lpart.get(), level_bits);
+      G500_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_level_report, dim3(1), dim3(kT), 0, s, (const uint64_t*)nullptr, 0,
+                         b->lpart.get(), (int)grid, ls, b->d_report, seq);
       G500_CHECK_LAUNCH();
       unv_ok = false;
     } else {
-      launch_bu(b, unv_ok, seq, level_bits);
+      launch_bu(b, unv_ok, seq, level_bits, true);
       unv_ok = true;
     }
     if (b->timing) G500_CHECK_HIP(hipEventRecord(lev1, s));
@@ -1382,10 +1466,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       // vertices the last levels reach are patched in at the end
       // (k_pred_fixup)
       const unsigned long long cseq = ++b->seq;
-      hipLaunchKernelGGL(k_bitmap_to_queue, dim3((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(words, kQueueWords))), dim3(kT),
-                         0, s, b->vis.get(), b->vis_alt.get(), words, g->deg_new.get(),
-                         b->q.get(), ls, b->d_report, cseq, b->vis_snap.get());
-      G500_CHECK_LAUNCH();
+      launch_to_queue(b, b->q.get(), cseq, b->vis_snap.get());
       G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
       G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
       launch_gather(b, root, b->vis_snap.get(), b->gstream);

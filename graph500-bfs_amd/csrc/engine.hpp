@@ -53,6 +53,9 @@ struct g500_bfs {
   g500::DevBuf<uint64_t> vis_snap;
   // without CoreBFS: original IDs of the vertices with edges, ascending (the
   // gather's domain; every other pred entry stays -1)
+  g500::DevBuf<uint64_t> lpart;   // per-block totals of a level's kernels
+  g500::DevBuf<uint32_t> qcnt;    // per-wave counts of the bitmap -> queue conversion
+  g500::DevBuf<uint64_t> qedges;  // its per-block degree sums
   g500::DevBuf<uint32_t> core_o;
   int64_t n_core = 0;
   int64_t iso_root = -1;  // a root without edges whose entry the last run wrote
