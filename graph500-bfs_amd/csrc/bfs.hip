@@ -1274,12 +1274,14 @@ void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_b
 // Frontier queue from the bits of the level just finished (vis minus
 // vis_alt, the bitmap before it); the level report carries its size and
 // degree sum.
+// With snap, ev_snap is recorded as soon as the snapshot is written.
 void launch_to_queue(g500_bfs* b, uint32_t* out, unsigned long long seq, uint64_t* snap) {
   hipStream_t s = b->stream;
   const dim3 grid((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(b->words, (int64_t)kT)));
   hipLaunchKernelGGL(k_queue_count, grid, dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(),
                      b->words, b->qcnt.get(), snap);
   G500_CHECK_LAUNCH();
+  if (snap) G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
   hipLaunchKernelGGL(k_queue_write, grid, dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(),
                      b->words, b->g->deg_new.get(), b->qcnt.get(), out, b->qedges.get());
   G500_CHECK_LAUNCH();
@@ -1467,7 +1469,6 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       // (k_pred_fixup)
       const unsigned long long cseq = ++b->seq;
       launch_to_queue(b, b->q.get(), cseq, b->vis_snap.get());
-      G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
       G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
       launch_gather(b, root, b->vis_snap.get(), b->gstream);
       G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
