@@ -72,14 +72,15 @@ __device__ inline uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
-// Sum over the block; valid in thread 0.
+// Sum over a block of W waves; valid in thread 0.
+template <int W = kWavesPerBlock>
 __device__ inline uint64_t block_sum(uint64_t v, uint64_t* lds) {
   v = wave_sum_u64(v);
   if (lane_id() == 0) lds[threadIdx.x / kWave] = v;
   __syncthreads();
   uint64_t s = 0;
   if (threadIdx.x == 0)
-    for (int w = 0; w < kWavesPerBlock; ++w) s += lds[w];
+    for (int w = 0; w < W; ++w) s += lds[w];
   return s;
 }
 
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(kT) void k_level_report(const uint64_t* __restrict_
   for (int i = threadIdx.x; i < nc; i += kT) c += cpart[i];
   for (int i = threadIdx.x; i < ne; i += kT) d += epart[i];
   c = block_sum(c, red);
+  __syncthreads();  // thread 0 has read red
   d = block_sum(d, red);
   if (threadIdx.x == 0) {
     unsigned long long v[4];
@@ -468,6 +470,9 @@ constexpr int kChunkVerts = kWave * 64;
 // The visited bits of the lowest new IDs (the highest degrees) sit in LDS:
 // most first neighbours are such hubs, so most probes never leave the CU.
 constexpr int kHubWords = 4096;
+// The bottom-up kernels' block size: all waves of a block share its hub copy.
+constexpr int kBuT = 256;
+constexpr int kBuWaves = kBuT / kWave;
 
 // Visited bit of u: from the block's LDS copy below hub_verts, else memory.
 struct VisProbe {
@@ -483,7 +488,7 @@ struct VisProbe {
 // Copies the first min(words, kHubWords) words of vis into LDS (all threads).
 __device__ inline VisProbe stage_hubs(uint64_t* hub, const uint64_t* vis, int64_t words) {
   const int hw = (int)(words < kHubWords ? words : kHubWords);
-  for (int i = threadIdx.x; i < hw; i += kT) hub[i] = vis[i];
+  for (int i = threadIdx.x; i < hw; i += kBuT) hub[i] = vis[i];
   __syncthreads();
   return VisProbe{hub, vis, (uint32_t)hw * 64u};
 }
@@ -567,7 +572,7 @@ __device__ inline bool multi_bit(const uint64_t* multi, uint32_t v) {
 constexpr int64_t kBuLongRow = 256;
 constexpr int kBuStep = 8;  // row entries a lane probes per step (two uint4 loads)
 
-__global__ __launch_bounds__(kT) void k_bu_sweep(
+__global__ __launch_bounds__(kBuT) void k_bu_sweep(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
@@ -575,15 +580,15 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
     uint32_t* __restrict__ unv_cnt, uint32_t* __restrict__ longv, LevelSync* ls,
     uint64_t* __restrict__ cpart, uint64_t level_bits) {
-  __shared__ unsigned long long fw[kWavesPerBlock][kWave];
-  __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ unsigned long long fw[kBuWaves][kWave];
+  __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t nchunks = ceil_div(words, kWave);
   uint64_t nfound = 0;
-  for (int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + wave; c < nchunks;
-       c += (int64_t)gridDim.x * kWavesPerBlock) {
+  for (int64_t c = (int64_t)blockIdx.x * kBuWaves + wave; c < nchunks;
+       c += (int64_t)gridDim.x * kBuWaves) {
     const int64_t w = c * kWave + lane;
     const uint64_t myword = w < words ? vis_old[w] : ~0ull;
     const uint64_t mymulti = w < words ? multi[w] : 0ull;
@@ -734,28 +739,28 @@ __global__ __launch_bounds__(kT) void k_bu_sweep(
     if (lane == 0) unv_cnt[c] = wp;
     wave_sync();
   }
-  const uint64_t s = block_sum(nfound, red);
+  const uint64_t s = block_sum<kBuWaves>(nfound, red);
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
 // The rows k_bu_sweep listed, a wave each: 256 entries per step, first
 // visited neighbour in row order; a vertex without one joins its chunk's
 // unvisited list.
-__global__ __launch_bounds__(kT) void k_bu_long(
+__global__ __launch_bounds__(kBuT) void k_bu_long(
     const uint64_t* __restrict__ vis_old, unsigned long long* vis_new, int64_t words,
     const int64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
     const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
     uint32_t* __restrict__ unv, uint32_t* unv_cnt,
     const uint32_t* __restrict__ longv, const LevelSync* ls,
     uint64_t* __restrict__ cpart, uint64_t level_bits) {
-  __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t n_long = (int64_t)ls->cnt[2];
   uint64_t nfound = 0;
-  for (int64_t it = (int64_t)blockIdx.x * kWavesPerBlock + wave; it < n_long;
-       it += (int64_t)gridDim.x * kWavesPerBlock) {
+  for (int64_t it = (int64_t)blockIdx.x * kBuWaves + wave; it < n_long;
+       it += (int64_t)gridDim.x * kBuWaves) {
     const uint32_t v = longv[it];
     int64_t k = row_off[v] + 1;
     const int64_t k1 = row_off[v + 1];
@@ -793,26 +798,26 @@ __global__ __launch_bounds__(kT) void k_bu_long(
       }
     }
   }
-  const uint64_t s = block_sum(nfound, red);
+  const uint64_t s = block_sum<kBuWaves>(nfound, red);
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kT) void k_bu_list(
+__global__ __launch_bounds__(kBuT) void k_bu_list(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
     uint32_t* __restrict__ unv_cnt, uint64_t* __restrict__ cpart, uint64_t level_bits) {
-  __shared__ unsigned long long fw[kWavesPerBlock][kWave];
-  __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ unsigned long long fw[kBuWaves][kWave];
+  __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t nchunks = ceil_div(words, kWave);
   uint64_t nfound = 0;
-  for (int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + wave; c < nchunks;
-       c += (int64_t)gridDim.x * kWavesPerBlock) {
+  for (int64_t c = (int64_t)blockIdx.x * kBuWaves + wave; c < nchunks;
+       c += (int64_t)gridDim.x * kBuWaves) {
     const int64_t w = c * kWave + lane;
     const uint64_t myword = w < words ? vis_old[w] : ~0ull;
     const uint32_t n = unv_cnt[c];
@@ -880,7 +885,7 @@ __global__ __launch_bounds__(kT) void k_bu_list(
     if (lane == 0) unv_cnt[c] = wp;
     wave_sync();
   }
-  const uint64_t s = block_sum(nfound, red);
+  const uint64_t s = block_sum<kBuWaves>(nfound, red);
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
@@ -983,6 +988,7 @@ __global__ __launch_bounds__(kT) void k_queue_report(const uint32_t* __restrict_
   for (int i = threadIdx.x; i < blocks * kWavesPerBlock; i += kT) c += wcnt[i];
   for (int i = threadIdx.x; i < blocks; i += kT) d += bedges[i];
   c = block_sum(c, red);
+  __syncthreads();  // thread 0 has read red
   d = block_sum(d, red);
   if (threadIdx.x == 0) {
     // a top-down level run as a sweep leaves the long-row count behind
@@ -1123,15 +1129,15 @@ void bfs_init(g500_bfs* b) {
   int cus = 0, per_cu[3] = {0, 0, 0};
   G500_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
   G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu[0], reinterpret_cast<const void*>(&k_bu_sweep), kT, 0));
+      &per_cu[0], reinterpret_cast<const void*>(&k_bu_sweep), kBuT, 0));
   G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu[1], reinterpret_cast<const void*>(&k_bu_list), kT, 0));
+      &per_cu[1], reinterpret_cast<const void*>(&k_bu_list), kBuT, 0));
   G500_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu[2], reinterpret_cast<const void*>(&k_bu_long), kT, 0));
+      &per_cu[2], reinterpret_cast<const void*>(&k_bu_long), kBuT, 0));
   for (int i = 0; i < 3; ++i)
     b->bu_grid[i] = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)std::max(cus, 1) * std::max(per_cu[i], 1),
-                             ceil_div(chunks, kWavesPerBlock)));
+                             ceil_div(chunks, kBuWaves)));
   b->lpart.alloc((size_t)std::max<int64_t>(
       kMaxGrid, std::max<int64_t>(b->bu_grid[0] + b->bu_grid[2], b->bu_grid[1])));
   b->lsync.alloc(1);
@@ -1242,20 +1248,20 @@ void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_b
   hipStream_t s = b->stream;
   int parts = 0;
   if (lists) {
-    hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kT), 0, s, b->vis.get(),
+    hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kBuT), 0, s, b->vis.get(),
                        b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
                        g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
                        b->unv.get(), b->unv_cnt.get(), b->lpart.get(), level_bits);
     G500_CHECK_LAUNCH();
     parts = (int)b->bu_grid[1];
   } else {
-    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kT), 0, s, b->vis.get(),
+    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kBuT), 0, s, b->vis.get(),
                        b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
                        g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
                        b->unv.get(), b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
                        b->lpart.get(), level_bits);
     G500_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kT), 0, s, b->vis.get(),
+    hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kBuT), 0, s, b->vis.get(),
                        (unsigned long long*)b->vis_alt.get(), b->words, g->row_off.get(),
                        g->col.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
                        b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
