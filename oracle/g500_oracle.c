@@ -585,6 +585,110 @@ int orc_do_bfs(const orc_graph* g, int64_t root, int ef, double alpha,
   return executed;
 }
 
+/* The same direction-optimizing BFS with OpenMP, as the reference's CPU path
+ * runs it: top-down threads split the frontier and claim children
+ * (top_down_search, bfs.hpp:1355-1395, per-thread queue buffers flushed to
+ * the shared next queue), bottom-up threads split the vertices
+ * (bottom_up_search_bitmap, bfs.hpp:2828-2978).  Frontier sets, and so
+ * at_data, dirs and depths, do not depend on thread order and equal
+ * orc_do_bfs's; a top-down parent is whichever frontier vertex claims the
+ * child first (bottom-up parents are the first visited neighbour in row
+ * order, as in orc_do_bfs). */
+#define ORC_OMP_BUF 4096
+int orc_do_bfs_omp(const orc_graph* g, int64_t root, int ef, double alpha,
+                   double beta, int64_t* pred, int64_t at[9], int8_t* dirs,
+                   int max_dirs, int nthreads) {
+  const int64_t n = g->n, nc = g->n_c;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t v = 0; v < n; ++v) pred[v] = -1;
+  for (int i = 0; i < 9; ++i) at[i] = -1;
+  if (root < 0 || root >= n) return 0;
+  uint8_t* vis = (uint8_t*)calloc((size_t)(nc ? nc : 1), 1);
+  uint8_t* vis_old = (uint8_t*)calloc((size_t)(nc ? nc : 1), 1);
+  uint32_t* cq = (uint32_t*)malloc((size_t)(nc ? nc : 1) * sizeof(uint32_t));
+  uint32_t* nq = (uint32_t*)malloc((size_t)(nc ? nc : 1) * sizeof(uint32_t));
+  const uint32_t r = g->orig2new[root];
+  int64_t cq_n = 0;
+  pred[root] = root;
+  if (r < (uint32_t)nc) {
+    vis[r] = 1;
+    cq[cq_n++] = r;
+  }
+  do_state s = {1, 1, 0};
+  int level = 0, executed = 0;
+  for (;;) {
+    at[8] = ++level;
+    const int td = s.next_td;
+    if (executed < max_dirs) dirs[executed] = (int8_t)td;
+    ++executed;
+    int64_t nq_n = 0, nq_edges = 0;
+    if (!td) {
+      const int64_t blk = 1 << 20;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+      for (int64_t b = 0; b < (nc + blk - 1) / blk; ++b) {
+        const int64_t lo = b * blk, len = (nc - lo < blk) ? nc - lo : blk;
+        memcpy(vis_old + lo, vis + lo, (size_t)len);
+      }
+    }
+#pragma omp parallel num_threads(nthreads) reduction(+ : nq_edges)
+    {
+      uint32_t buf[ORC_OMP_BUF];
+      int nb = 0;
+#define ORC_FLUSH()                                                         \
+  do {                                                                      \
+    const int64_t at_ = __atomic_fetch_add(&nq_n, (int64_t)nb, __ATOMIC_RELAXED); \
+    memcpy(nq + at_, buf, (size_t)nb * sizeof(uint32_t));                   \
+    nb = 0;                                                                 \
+  } while (0)
+      if (td) {
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t i = 0; i < cq_n; ++i) {
+          const uint32_t u = cq[i];
+          for (int64_t e = g->row_off[u]; e < g->row_off[u + 1]; ++e) {
+            const uint32_t v = g->col[e];
+            if (__atomic_load_n(&vis[v], __ATOMIC_RELAXED)) continue;
+            if (__atomic_exchange_n(&vis[v], (uint8_t)1, __ATOMIC_RELAXED)) continue;
+            pred[g->new2orig[v]] = PRED_ENTRY(g->new2orig[u], level);
+            nq_edges += g->deg_new[v];
+            buf[nb++] = v;
+            if (nb == ORC_OMP_BUF) ORC_FLUSH();
+          }
+        }
+      } else {
+#pragma omp for schedule(dynamic, 4096)
+        for (int64_t v = 0; v < nc; ++v) {
+          if (vis_old[v]) continue;
+          for (int64_t e = g->row_off[v]; e < g->row_off[v + 1]; ++e) {
+            const uint32_t u = g->col[e];
+            if (!vis_old[u]) continue;
+            vis[v] = 1;
+            pred[g->new2orig[v]] = PRED_ENTRY(g->new2orig[u], level);
+            nq_edges += g->deg_new[v];
+            buf[nb++] = (uint32_t)v;
+            if (nb == ORC_OMP_BUF) ORC_FLUSH();
+            break;
+          }
+        }
+      }
+      if (nb) ORC_FLUSH();
+#undef ORC_FLUSH
+    }
+    if (nq_n == 0) break;
+    do_decide(&s, td, nq_n, nq_edges, g->m, nc, ef, alpha, beta, level, at);
+    s.prev = nq_n;
+    uint32_t* t = cq;
+    cq = nq;
+    nq = t;
+    cq_n = nq_n;
+  }
+  free(vis);
+  free(vis_old);
+  free(cq);
+  free(nq);
+  return executed;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Validation (validate.hpp:608-1255), single rank                           */
 /* ------------------------------------------------------------------------ */

@@ -112,6 +112,13 @@ int orc_do_bfs(const orc_graph* g, int64_t root, int edgefactor, double alpha,
                double beta, int64_t* pred_by_orig, int64_t at_data[9],
                int8_t* dirs, int max_dirs);
 
+/* orc_do_bfs with OpenMP over nthreads threads (the reference's CPU path
+ * parallelises its sweeps the same way, bfs.hpp:1355-1395, 2828-2978).  Same
+ * at_data, dirs and depths; top-down parents may differ (first claimer). */
+int orc_do_bfs_omp(const orc_graph* g, int64_t root, int edgefactor, double alpha,
+                   double beta, int64_t* pred_by_orig, int64_t at_data[9],
+                   int8_t* dirs, int max_dirs, int nthreads);
+
 /* Replays only the alpha/beta decision logic of bfs.hpp:3298-3457 given the
  * per-level frontier sizes and frontier degree sums (index 1..nlev-1 are
  * levels 1..; index 0 unused).  Same outputs as orc_do_bfs. */

@@ -75,6 +75,8 @@ def lib():
         L.orc_bfs.argtypes = [C.c_void_p, C.c_int64, i32p, i64p]
         L.orc_do_bfs.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_double, i64p,
                                  i64p, C.POINTER(C.c_int8), C.c_int]
+        L.orc_do_bfs_omp.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_double,
+                                     i64p, i64p, C.POINTER(C.c_int8), C.c_int, C.c_int]
         L.orc_do_schedule.argtypes = [C.c_int64, C.c_int64, C.c_int, C.c_double, C.c_double,
                                       C.c_int, i64p, i64p, C.c_int, i64p,
                                       C.POINTER(C.c_int8), C.c_int]
@@ -175,6 +177,14 @@ class Graph:
         dirs = np.zeros(256, np.int8)
         k = lib().orc_do_bfs(self.h, root, ef, alpha, beta, _p(pred, C.c_int64),
                              _p(at, C.c_int64), _p(dirs, C.c_int8), 256)
+        return pred, at, dirs[:min(k, 256)].copy()
+
+    def do_bfs_omp(self, root, nthreads, ef=16, alpha=10.0, beta=14.0):
+        pred = np.empty(self.info.n, np.int64)
+        at = np.empty(9, np.int64)
+        dirs = np.zeros(256, np.int8)
+        k = lib().orc_do_bfs_omp(self.h, root, ef, alpha, beta, _p(pred, C.c_int64),
+                                 _p(at, C.c_int64), _p(dirs, C.c_int8), 256, nthreads)
         return pred, at, dirs[:min(k, 256)].copy()
 
     def find_roots(self, nroots, r1=0, r2=0):

@@ -195,6 +195,25 @@ def test_do_bfs_levels_equal_textbook():
         assert ok, errs
 
 
+def test_do_bfs_omp_matches_serial():
+    """The OpenMP restatement (the CPU baseline) keeps the serial one's
+    schedule, at_data and depths; its parents pass the validator.  Alpha and
+    beta extremes force pure top-down and never-return runs too."""
+    v0, v1 = O.generate_edges(15)
+    g = O.Graph(v0, v1)
+    for alpha, beta in ((10.0, 14.0), (1e9, 14.0), (10.0, 1e-6)):
+        for r in g.find_roots(4):
+            r = int(r)
+            pred_s, at_s, dirs_s = g.do_bfs(r, alpha=alpha, beta=beta)
+            for nt in (1, 4):
+                pred, at, dirs = g.do_bfs_omp(r, nt, alpha=alpha, beta=beta)
+                assert at.tolist() == at_s.tolist() and dirs.tolist() == dirs_s.tolist()
+                depth = lambda p: np.where(p < 0, -1, (p.astype(np.uint64) >> np.uint64(48)).astype(np.int64))  # noqa: E731
+                assert np.array_equal(depth(pred), depth(pred_s))
+                ok, te, errs = g.validate(r, pred)
+                assert ok, errs
+
+
 # ------------------------------------------------------- DO schedule
 def test_do_schedule_hand_traced():
     # m = 1000 edges, alpha 10 -> switch to BU when a growing TD level has
