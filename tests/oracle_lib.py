@@ -143,6 +143,25 @@ class Graph:
         self.info = GraphInfo()
         lib().orc_graph_get_info(self.h, C.byref(self.info))
 
+    @classmethod
+    def wrap(cls, log_n, n_c, m, csr):
+        """The oracle over CSR arrays built elsewhere (row_off, col, deg_new,
+        new2orig, orig2new as int64/uint32 numpy arrays, kept alive here)."""
+        self = cls.__new__(cls)
+        self.v0 = self.v1 = None
+        self._keep = {k: np.ascontiguousarray(csr[k], np.int64 if k in ("row_off", "deg_new")
+                                               else np.uint32)
+                      for k in ("row_off", "col", "deg_new", "new2orig", "orig2new")}
+        k = self._keep
+        self.h = lib().orc_graph_wrap(log_n, n_c, m, _p(k["row_off"], C.c_int64),
+                                      _p(k["col"], C.c_uint32), _p(k["deg_new"], C.c_int64),
+                                      _p(k["new2orig"], C.c_uint32), _p(k["orig2new"], C.c_uint32))
+        if not self.h:
+            raise MemoryError("orc_graph_wrap failed")
+        self.info = GraphInfo()
+        lib().orc_graph_get_info(self.h, C.byref(self.info))
+        return self
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_graph_free(self.h)

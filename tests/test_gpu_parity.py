@@ -94,6 +94,30 @@ def test_bfs_scale20_te(G, golden):
     _check_roots(G, g, og, roots, golden_te=golden["pf_nedge"][20])
 
 
+def test_bfs_scale24_drop_in(G, golden):
+    """The SCALE 24 drop-in case: the reference's logged edge counts, 64 roots
+    by the reference's rule, every parent tree valid with the reference's
+    traversed-edge count, and for four roots levels bit-exact against the
+    oracle's textbook BFS over the same CSR (the CSR build itself is compared
+    with the oracle's at SCALE 16)."""
+    print("\n[bfs] scale=24 roots=64", flush=True)
+    g = G.Graph.generate(24).construct()
+    inf = g.info
+    ref = golden["edge_counts"]["24"]
+    assert (inf.m_sym, inf.m) == (ref["sym"], ref["dedup"])
+    roots = g.find_roots(64)
+    assert len(roots) == 64
+    bfs = G.Bfs(g)
+    for r in roots.tolist():
+        bfs.run(r)
+        ok, te, errs = bfs.validate(r)
+        assert ok and te == golden["pf_nedge"][24], (r, errs, te)
+    og = O.Graph.wrap(inf.log_n, inf.n_c, inf.m, g.csr())
+    for r in roots[:4].tolist():
+        bfs.run(r)
+        assert np.array_equal(bfs.levels(), _oracle_levels(og, r)), f"levels differ for root {r}"
+
+
 @pytest.mark.parametrize("alpha,beta", [(1e9, 14.0), (1e-3, 14.0), (10.0, 1e-6), (0.5, 1e6)])
 def test_direction_extremes(G, alpha, beta):
     """Pure top-down, immediate bottom-up, never-return and eager-return."""
