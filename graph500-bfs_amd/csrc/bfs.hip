@@ -802,7 +802,10 @@ __global__ __launch_bounds__(kBuT) void k_bu_long(
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kBuT) void k_bu_list(
+// Four waves per SIMD (128 VGPRs, a 12-byte spill) rather than the three its
+// registers would give: its LDS allows four, and the list levels are
+// latency-bound.
+__global__ __launch_bounds__(kBuT) __attribute__((amdgpu_waves_per_eu(4))) void k_bu_list(
     const uint64_t* __restrict__ vis_old, uint64_t* __restrict__ vis_new,
     int64_t words, const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
