@@ -231,47 +231,44 @@ __global__ __launch_bounds__(kT) void k_ms_prep(MsLevel a, int64_t bm_words) {
       }
 }
 
-// Top-down: push f to w (lanes not yet seen at w).
+// Top-down: push f to w (lanes not yet seen at w).  The ORs return nothing,
+// so a push waits on no atomic: the level's counts (new pairs, frontier
+// degree sum) are taken afterwards from F_{L+1} by k_ms_td_count.  The
+// bitmap word is read beside seen so a vertex already in the next frontier
+// costs no second atomic.
 template <int K>
-__device__ inline void ms_push(const MsLevel& a, uint32_t w, const uint64_t (&f)[K],
-                               uint64_t& disc, uint64_t& edges) {
+__device__ inline void ms_push(const MsLevel& a, uint32_t w, const uint64_t (&f)[K]) {
   uint64_t sw[K];
   load_lanes<K>(a.seen + (int64_t)w * K, sw);
-  bool opened = false;
+  const uint64_t bit = 1ull << (w & 63);
+  const bool listed = (__hip_atomic_load(&a.fbn[w >> 6], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) & bit) != 0;
+  bool any = false;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint64_t m = f[k] & ~sw[k];
     if (m) {
-      const unsigned long long old =
-          atomicOr((unsigned long long*)&a.fn[(int64_t)w * K + k], (unsigned long long)m);
-      const uint64_t nb = m & ~(uint64_t)old;
-      disc += (uint64_t)__popcll(nb);
-      opened = opened || (nb != 0 && old == 0);
+      atomicOr((unsigned long long*)&a.fn[(int64_t)w * K + k], (unsigned long long)m);
+      any = true;
     }
   }
-  if (opened) {
-    const unsigned long long bit = 1ull << (w & 63);
-    if (!(atomicOr((unsigned long long*)&a.fbn[w >> 6], bit) & bit)) edges += a.deg[w];
-  }
+  if (any && !listed) atomicOr((unsigned long long*)&a.fbn[w >> 6], (unsigned long long)bit);
 }
 
 // Top-down over new IDs [vb, ve), G lanes per vertex.
 template <int K, int G>
 __global__ __launch_bounds__(kT) void k_ms_td(MsLevel a, int64_t vb, int64_t ve) {
-  __shared__ uint64_t red[kWavesPerBlock];
   if (!ms_active(a) || !ms_top_down(a, a.L)) return;
   const int lig = threadIdx.x % G;
   const int64_t gid = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
   const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
-  uint64_t disc = 0, edges = 0;
   for (int64_t v = vb + gid; v < ve; v += ngroups) {
     if (!in_bitmap(a.fb, v)) continue;
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
     const int64_t end = a.row_off[v + 1];
-    for (int64_t e = a.row_off[v] + lig; e < end; e += G) ms_push<K>(a, ld_col(a.col + e), f, disc, edges);
+    for (int64_t e = a.row_off[v] + lig; e < end; e += G) ms_push<K>(a, ld_col(a.col + e), f);
   }
-  ms_flush(a, red, disc, edges);
 }
 
 // Top-down for heavy vertices: one wave per work item of `chunk` edges.
@@ -280,12 +277,10 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
                                                    const uint64_t* __restrict__ item_start,
                                                    const uint32_t* __restrict__ item_vertex,
                                                    int64_t n_items, int chunk) {
-  __shared__ uint64_t red[kWavesPerBlock];
   if (!ms_active(a) || !ms_top_down(a, a.L)) return;
   const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
-  uint64_t disc = 0, edges = 0;
   for (int64_t it = wid; it < n_items; it += nw) {
     const int64_t v = item_vertex[it];
     if (!in_bitmap(a.fb, v)) continue;
@@ -293,7 +288,26 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
     load_lanes<K>(a.fl + v * K, f);
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e = beg + lane; e < end; e += kWave) ms_push<K>(a, ld_col(a.col + e), f, disc, edges);
+    for (int64_t e = beg + lane; e < end; e += kWave) ms_push<K>(a, ld_col(a.col + e), f);
+  }
+}
+
+// After a top-down level's pushes: new (vertex, root) pairs = the lanes of
+// F_{L+1} (pushes left out lanes already seen), frontier degree sum = the
+// degrees of the vertices in its bitmap.  One thread per vertex.
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_td_count(MsLevel a) {
+  __shared__ uint64_t red[kWavesPerBlock];
+  if (!ms_active(a) || !ms_top_down(a, a.L)) return;
+  uint64_t disc = 0, edges = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < a.n_c;
+       v += (int64_t)gridDim.x * kT) {
+    if (!in_bitmap(a.fbn, v)) continue;
+    uint64_t x[K];
+    load_lanes<K>(a.fn + v * K, x);
+#pragma unroll
+    for (int k = 0; k < K; ++k) disc += (uint64_t)__popcll(x[k]);
+    edges += a.deg[v];
   }
   ms_flush(a, red, disc, edges);
 }
@@ -1197,6 +1211,8 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
     default: launch_td<K, 32>(b, a, h, m, s); break;
   }
   launch_td<K, 1>(b, a, m, n_c, s);
+  hipLaunchKernelGGL((k_ms_td_count<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a);
+  G500_CHECK_LAUNCH();
   if (b->g->m > 0) {
     const int64_t ntiles = ceil_div(b->g->m, StreamTile<K>::kTile);
     const unsigned sgrid = (unsigned)std::max<int64_t>(
