@@ -54,6 +54,8 @@ def parse():
                     help="batched leg with the default tuning (no autotuning; for profiling)")
     ap.add_argument("--serve-queries", type=int, default=4000)
     ap.add_argument("--serve-qps", type=float, default=2000.0)
+    ap.add_argument("--serve-engines", type=int, default=1,
+                    help="batched engines per resident graph in the serving loop")
     return ap.parse_args()
 
 
@@ -191,8 +193,11 @@ def serving(g500bfs, args):
         g = g500bfs.Graph.generate(scale, 16, 2 + i, 3 + i).construct()
         graphs.append(g)
         pools.append(g.find_roots(512))
-    srv = serve.Server(graphs, max_batch=256)
-    srv.prewarm(4, pools)
+    # one engine per graph: the device is bandwidth-bound, so overlapping
+    # batches of one graph only makes them smaller (measured: 1 765 q/s,
+    # p50 ~310 ms with one engine against 1 660 q/s, p50 ~520 ms with four)
+    srv = serve.Server(graphs, max_batch=256, max_engines_per_graph=args.serve_engines)
+    srv.prewarm(args.serve_engines, pools)
     qs = serve.poisson_workload(graphs, pools, args.serve_queries, args.serve_qps, seed=7)
     t0 = time.perf_counter()
     done = serve.run_open_loop(srv, qs)
