@@ -54,7 +54,7 @@ def parse():
                     help="batched leg with the default tuning (no autotuning; for profiling)")
     ap.add_argument("--serve-queries", type=int, default=4000)
     ap.add_argument("--serve-qps", type=float, default=2000.0)
-    ap.add_argument("--serve-engines", type=int, default=1,
+    ap.add_argument("--serve-engines", type=int, default=2,
                     help="batched engines per resident graph in the serving loop")
     return ap.parse_args()
 
@@ -193,9 +193,8 @@ def serving(g500bfs, args):
         g = g500bfs.Graph.generate(scale, 16, 2 + i, 3 + i).construct()
         graphs.append(g)
         pools.append(g.find_roots(512))
-    # one engine per graph: the device is bandwidth-bound, so overlapping
-    # batches of one graph only makes them smaller (measured: 1 765 q/s,
-    # p50 ~310 ms with one engine against 1 660 q/s, p50 ~520 ms with four)
+    # two engines per graph: the server overlaps a second batch of a graph
+    # only when a full one is waiting, so more engines add no small batches
     srv = serve.Server(graphs, max_batch=256, max_engines_per_graph=args.serve_engines)
     srv.prewarm(args.serve_engines, pools)
     qs = serve.poisson_workload(graphs, pools, args.serve_queries, args.serve_qps, seed=7)

@@ -8,7 +8,11 @@ flight.  The loop
 
   * admits arriving queries into a per-graph queue,
   * launches a batch on an idle engine as soon as its graph has queued
-    queries (up to max_batch of them, never waiting for a batch to fill),
+    queries and none of its batches is in flight (up to max_batch of them,
+    never waiting for a batch to fill); while one is in flight, another
+    batch of the same graph starts only when a full one (max_batch queries)
+    is waiting -- overlapping batches of one graph share the device's
+    bandwidth, so small ones would only shrink every batch,
   * creates another engine for a graph whose engines are all busy while the
     free-device-memory budget covers one more engine's footprint and still
     leaves room for a first engine of every graph that has none,
@@ -108,6 +112,9 @@ class Server:
     def _launch(self, now):
         for gid, qu in enumerate(self.queues):
             while qu:
+                busy = any(e.batch is not None for e in self.engines[gid])
+                if busy and len(qu) < self.max_batch:
+                    break  # the running batch finishes first; a fuller one follows
                 e = self._idle_engine(gid)
                 if e is None:
                     break

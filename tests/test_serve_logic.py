@@ -81,6 +81,25 @@ def test_continuous_batching_uses_more_engines_while_others_run():
     assert len(srv.done) == 6
 
 
+def test_partial_batch_waits_while_its_graph_runs():
+    """While a batch of a graph is in flight, a second batch of that graph
+    starts only when a full one is waiting; a partial queue waits for the
+    running batch and then goes out at once."""
+    srv = make_server(ngraphs=1, max_batch=4, budget=30, footprint=10, engines=3)
+    for i in range(6):
+        srv.submit(serve.Query(i, 0, i, [0], 0.0))
+    srv.poll(clock=lambda: 0.0)   # one batch of 4; the 2 left wait
+    assert [n for _, n in FakeEngine.launches] == [4]
+    for i in range(6, 9):        # 5 queued: a full batch overlaps
+        srv.submit(serve.Query(i, 0, i, [0], 0.0))
+    srv.poll(clock=lambda: 0.0)
+    assert [n for _, n in FakeEngine.launches] == [4, 4]
+    assert sum(e.batch is not None for e in srv.engines[0]) == 2
+    srv.drain(clock=lambda: 0.0)
+    assert [n for _, n in FakeEngine.launches] == [4, 4, 1]
+    assert len(srv.done) == 9
+
+
 def test_extra_engines_leave_room_for_idle_graphs():
     srv = make_server(ngraphs=3, max_batch=1, budget=30, footprint=10, engines=4)
     for i in range(5):
