@@ -150,11 +150,12 @@ __global__ __launch_bounds__(kT) void k_level_report(const uint64_t* __restrict_
 // Bits of vertices >= n_c are pre-set so they are never searched
 // (the reference pre-visits vertices without edges, mpi/bfs.hpp:330-334).
 // Per-root init in one launch: the bitmap (plus the root's bit) and, from
-// thread 0, the root's pred entry and the one-vertex frontier.
+// thread 0, the root's pred entry, the one-vertex frontier and cleared level
+// counters (a run that failed inside a level leaves its reservations there).
 __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
                                                 int64_t words, int64_t n_c,
                                                 int64_t root_new, int64_t* pred_new,
-                                                int64_t entry, uint32_t* q) {
+                                                int64_t entry, uint32_t* q, LevelSync* ls) {
   for (int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x; w < words;
        w += (int64_t)gridDim.x * kT) {
     const int64_t lo = w * 64;
@@ -164,9 +165,13 @@ __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
     if (root_new >= lo && root_new < lo + 64) word |= 1ull << (root_new - lo);
     vis[w] = word;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && root_new >= 0 && root_new < n_c) {
-    pred_new[root_new] = entry;
-    q[0] = (uint32_t)root_new;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ls->cnt[i] = 0;
+    if (root_new >= 0 && root_new < n_c) {
+      pred_new[root_new] = entry;
+      q[0] = (uint32_t)root_new;
+    }
   }
 }
 
@@ -1357,7 +1362,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1,
                      b->pred_new.get(),
-                     (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get());
+                     (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get(),
+                     b->lsync.get());
   G500_CHECK_LAUNCH();
 
   int64_t qlen = root_in ? 1 : 0;
