@@ -5,12 +5,12 @@
 
 #include "internal.hpp"
 
-// Device-side per-level counters; reset by the last workgroup of the kernel
-// that closes a level, after it has posted the totals to the host.
+// Device-side counters that queue reservations take within a level
+// (top-down next-queue slots in cnt[0], long-row slots in cnt[2]).  The
+// level's one-block report kernel adds them to the blocks' totals it posts
+// and clears them; the per-root init clears them too.
 struct LevelSync {
   unsigned long long cnt[4];
-  unsigned int done;
-  unsigned int pad;
 };
 
 // Host-visible (pinned, coherent, mapped) level report.
@@ -47,15 +47,16 @@ struct g500_bfs {
 
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   // the pred gather of the vertices reached by the last bottom-up level runs
-  // on gstream during the remaining top-down levels, against vis_snap
+  // on gstream during the remaining top-down levels, against vis_snap (the
+  // bitmap copy the conversion's count pass writes)
   hipStream_t gstream = nullptr;
   hipEvent_t ev_snap = nullptr, ev_gather = nullptr;
   g500::DevBuf<uint64_t> vis_snap;
-  // without CoreBFS: original IDs of the vertices with edges, ascending (the
-  // gather's domain; every other pred entry stays -1)
   g500::DevBuf<uint64_t> lpart;   // per-block totals of a level's kernels
   g500::DevBuf<uint32_t> qcnt;    // per-wave counts of the bitmap -> queue conversion
   g500::DevBuf<uint64_t> qedges;  // its per-block degree sums
+  // without CoreBFS: original IDs of the vertices with edges, ascending (the
+  // gather's domain; every other pred entry stays -1)
   g500::DevBuf<uint32_t> core_o;
   int64_t n_core = 0;
   int64_t iso_root = -1;  // a root without edges whose entry the last run wrote
