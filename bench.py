@@ -141,9 +141,22 @@ def cpu_baseline(graph, roots, te_by_root, budget_s, args):
 
 def batched(g500bfs, graph, bfs, args, plain_teps):
     """Batched bit-parallel BFS on the same graph: tuned per batch size on one
-    root set, timed on another, levels spot-checked against the plain engine."""
+    root set, the tuned configuration kept only if it also beats the default
+    on a second set, timed on a third; levels spot-checked against the plain
+    engine."""
     mb = g500bfs.MultiBfs(graph, max_roots=256)
     tune_roots = graph.find_roots(256, r1=101)
+    check_roots = graph.find_roots(256, r1=202)
+
+    def batch_ms(tuning, r):
+        mb.set_tuning(**tuning)
+        mb.run(r)
+        best = float("inf")
+        for _ in range(2):
+            mb.run(r)
+            best = min(best, mb.stats()["gpu_ms"])
+        return best
+
     default = mb.tuning()
     roots = graph.find_roots(256, r1=0)
     out = {}
@@ -153,7 +166,13 @@ def batched(g500bfs, graph, bfs, args, plain_teps):
             tuning = default
         else:
             tuning, tune_ms = mb.autotune(tune_roots[:nr], rounds=1)
-            log(f"batched {nr}: tuning {tuning} ({tune_ms:.2f} ms per batch of the tuning roots)")
+            t_tuned = batch_ms(tuning, check_roots[:nr])
+            t_default = batch_ms(default, check_roots[:nr])
+            log(f"batched {nr}: tuning {tuning} ({tune_ms:.2f} ms per batch of the tuning roots; "
+                f"second set: {t_tuned:.2f} ms tuned, {t_default:.2f} ms default)")
+            if t_default < t_tuned:
+                tuning = default
+            mb.set_tuning(**tuning)
         r = roots[:nr]
         mb.run(r)
         host, dev = [], []
