@@ -94,7 +94,8 @@ __global__ __launch_bounds__(kT) void k_ms_init(const int64_t* __restrict__ d_ro
                                                uint64_t* __restrict__ acc, int64_t acc_words,
                                                unsigned long long* __restrict__ cnt,
                                                unsigned long long* __restrict__ fe, int ncnt,
-                                               unsigned long long* __restrict__ te) {
+                                               unsigned long long* __restrict__ te,
+                                               uint64_t* __restrict__ act) {
   const int nroots = (int)d_roots[0];
   const int64_t words = n_c * K;
   const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -112,6 +113,7 @@ __global__ __launch_bounds__(kT) void k_ms_init(const int64_t* __restrict__ d_ro
     cnt[j] = 0;
     fe[j] = 0;
   }
+  for (int64_t j = i0; j < (int64_t)ncnt * 4; j += stride) act[j] = 0;
   for (int64_t j = i0; j < G500_MSBFS_MAX_ROOTS; j += stride) te[j] = 0;
 }
 
@@ -120,7 +122,7 @@ __global__ void k_ms_roots(const int64_t* __restrict__ d_roots,
                            const uint32_t* __restrict__ deg, int64_t n_c, int K,
                            uint64_t* __restrict__ seen, uint64_t* __restrict__ f0,
                            uint64_t* __restrict__ fb0, unsigned long long* cnt,
-                           unsigned long long* fe) {
+                           unsigned long long* fe, uint64_t* act) {
   const int nroots = (int)d_roots[0];
   const int i = threadIdx.x;
   if (i >= nroots) return;
@@ -129,6 +131,7 @@ __global__ void k_ms_roots(const int64_t* __restrict__ d_roots,
   const uint64_t bit = 1ull << (i & 63);
   atomicOr((unsigned long long*)&f0[v * K + (i >> 6)], (unsigned long long)bit);
   atomicOr((unsigned long long*)&seen[v * K + (i >> 6)], (unsigned long long)bit);
+  atomicOr((unsigned long long*)&act[i >> 6], (unsigned long long)bit);
   const unsigned long long vb = 1ull << (v & 63);
   if (!(atomicOr((unsigned long long*)&fb0[v >> 6], vb) & vb)) atomicAdd(&fe[0], (unsigned long long)deg[v]);
   atomicAdd(&cnt[0], 1ull);
@@ -172,6 +175,8 @@ struct MsLevel {
   uint64_t* fbn;        // bitmap of F_{L+1}
   unsigned long long* cnt;
   unsigned long long* fe;
+  const uint64_t* act_l;   // lanes present anywhere in F_L (4 words)
+  uint64_t* act_n;         // the same for F_{L+1}, ORed by the kernels that write it
   unsigned long long thr;
   const int64_t* d_roots;  // [0] = batch size
   double stream_frac;
@@ -206,6 +211,22 @@ __device__ inline void ms_flush(const MsLevel& a, uint64_t* red, uint64_t disc, 
   if (threadIdx.x == 0) {
     if (d) atomicAdd(&a.cnt[a.L + 1], (unsigned long long)d);
     if (e) atomicAdd(&a.fe[a.L + 1], (unsigned long long)e);
+  }
+}
+
+// ORs the lanes a thread wrote to F_{L+1} into act_n: a wave reduction, then
+// one atomic per word per wave, skipped when the word already holds them.
+template <int K>
+__device__ inline void ms_note_lanes(const MsLevel& a, uint64_t (&lanes)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lanes[k] |= __shfl_xor(lanes[k], o, kWave);
+    if (lane_id() == 0 && lanes[k]) {
+      const uint64_t have = __hip_atomic_load(&a.act_n[k], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+      if (lanes[k] & ~have) atomicOr((unsigned long long*)&a.act_n[k], (unsigned long long)lanes[k]);
+    }
   }
 }
 
@@ -299,16 +320,22 @@ template <int K>
 __global__ __launch_bounds__(kT) void k_ms_td_count(MsLevel a) {
   __shared__ uint64_t red[kWavesPerBlock];
   if (!ms_active(a) || !ms_top_down(a, a.L)) return;
-  uint64_t disc = 0, edges = 0;
+  uint64_t disc = 0, edges = 0, lanes[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) lanes[k] = 0;
   for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < a.n_c;
        v += (int64_t)gridDim.x * kT) {
     if (!in_bitmap(a.fbn, v)) continue;
     uint64_t x[K];
     load_lanes<K>(a.fn + v * K, x);
 #pragma unroll
-    for (int k = 0; k < K; ++k) disc += (uint64_t)__popcll(x[k]);
+    for (int k = 0; k < K; ++k) {
+      disc += (uint64_t)__popcll(x[k]);
+      lanes[k] |= x[k];
+    }
     edges += a.deg[v];
   }
+  ms_note_lanes<K>(a, lanes);
   ms_flush(a, red, disc, edges);
 }
 
@@ -371,7 +398,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_stream(MsLevel a, int64_t m,
       load_lanes<K>(a.fl + (v0 + i) * K, f);
       bool full = true;
 #pragma unroll
-      for (int k = 0; k < K; ++k) full = full && ((s0[k] | f[k]) == ~0ull);
+      for (int k = 0; k < K; ++k) full = full && ((s0[k] | f[k] | ~a.act_l[k]) == ~0ull);
       full_l[i] = full ? 1 : 0;
     }
     __syncthreads();
@@ -474,7 +501,7 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_pull(MsLevel a, int64_t vb, int64
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       s[k] |= f[k];
-      full = full && (s[k] == ~0ull);
+      full = full && ((s[k] | ~a.act_l[k]) == ~0ull);
     }
     if (full) continue;
     const int64_t beg = a.row_off[v], end = a.row_off[v + 1];
@@ -540,7 +567,7 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_pull_heavy(MsLevel a,
     load_lanes<K>(a.fl + v * K, f);
     bool full = true;
 #pragma unroll
-    for (int k = 0; k < K; ++k) full = full && ((s[k] | f[k]) == ~0ull);
+    for (int k = 0; k < K; ++k) full = full && ((s[k] | f[k] | ~a.act_l[k]) == ~0ull);
     if (full) continue;
     uint64_t acc[K];
 #pragma unroll
@@ -623,7 +650,9 @@ template <int K>
 __global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a) {
   __shared__ uint64_t red[kWavesPerBlock];
   if (!ms_active(a) || ms_mode(a, a.L) != kModeStream) return;
-  uint64_t disc = 0, edges = 0;
+  uint64_t disc = 0, edges = 0, lanes[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) lanes[k] = 0;
   const int64_t stride = (int64_t)gridDim.x * kT;
   for (int64_t vbase = (int64_t)blockIdx.x * kT + (threadIdx.x / kWave) * kWave; vbase < a.n_c;
        vbase += stride) {
@@ -641,6 +670,7 @@ __global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a) {
         if (nxt != x[k]) a.fn[v * K + k] = nxt;
         if ((s | nxt) != s0[k]) a.seen[v * K + k] = s | nxt;
         disc += (uint64_t)__popcll(nxt);
+        lanes[k] |= nxt;
         nz = nz || nxt != 0;
       }
       if (nz) edges += a.deg[v];
@@ -649,6 +679,7 @@ __global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a) {
     const uint64_t word = __ballot(nz);
     if (lane_id() == 0 && word) atomicOr((unsigned long long*)&a.fbn[vbase >> 6], (unsigned long long)word);
   }
+  ms_note_lanes<K>(a, lanes);
   ms_flush(a, red, disc, edges);
 }
 
@@ -663,6 +694,15 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
   const int64_t gid0 = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
   const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
   uint64_t disc = 0, edges = 0;
+  // a lane absent from F_L cannot be found this level: it counts as covered
+  // when deciding that a vertex is done (roots whose search has ended, e.g.
+  // in a small component, would otherwise keep every row scanning to its end)
+  uint64_t inact[K], lanes[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    inact[k] = ~a.act_l[k];
+    lanes[k] = 0;
+  }
   // the loop runs wave-uniformly (lanes past ve idle) so that the frontier
   // bitmap bits of a wave's vertices can be combined with shuffles
   const int64_t lane_span = (int64_t)kWave / G;  // vertices per wave
@@ -682,7 +722,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         s[k] = s0[k] | f[k];
-        full = full && (s[k] == ~0ull);
+        full = full && ((s[k] | inact[k]) == ~0ull);
       }
       if (!full) {
         const int64_t beg = a.row_off[v], end = a.row_off[v + 1];
@@ -713,7 +753,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
           }
           bool cov = true;
 #pragma unroll
-          for (int k = 0; k < K; ++k) cov = cov && ((acc[k] | s[k]) == ~0ull);
+          for (int k = 0; k < K; ++k) cov = cov && ((acc[k] | s[k] | inact[k]) == ~0ull);
           if (cov) break;
         }
 #pragma unroll
@@ -729,6 +769,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
           const uint64_t ns = s[k] | nxt[k];
           if (ns != s0[k]) a.seen[v * K + k] = ns;
           disc += (uint64_t)__popcll(nxt[k]);
+          lanes[k] |= nxt[k];
         }
         if (nz) edges += a.deg[v];
       }
@@ -749,6 +790,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
     }
     (void)lane_span;
   }
+  ms_note_lanes<K>(a, lanes);
   ms_flush(a, red, disc, edges);
 }
 
@@ -765,6 +807,9 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy(MsLevel a,
   const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+  uint64_t inact[K];  // lanes absent from F_L count as covered (as in k_ms_bu)
+#pragma unroll
+  for (int k = 0; k < K; ++k) inact[k] = ~a.act_l[k];
   for (int64_t it = wid; it < n_items; it += nw) {
     const int64_t v = item_vertex[it];
     uint64_t s[K], g0[K], f[K];
@@ -775,7 +820,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy(MsLevel a,
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       s[k] |= f[k];
-      full = full && ((s[k] | g0[k]) == ~0ull);
+      full = full && ((s[k] | g0[k] | inact[k]) == ~0ull);
     }
     if (full) continue;
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
@@ -805,7 +850,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy(MsLevel a,
         for (int o = 32; o > 0; o >>= 1) acc[k] |= __shfl_xor(acc[k], o, kWave);
       bool cov = true;
 #pragma unroll
-      for (int k = 0; k < K; ++k) cov = cov && ((acc[k] | s[k] | g0[k]) == ~0ull);
+      for (int k = 0; k < K; ++k) cov = cov && ((acc[k] | s[k] | g0[k] | inact[k]) == ~0ull);
       if (cov) break;
     }
     if (lane == 0) {
@@ -823,7 +868,9 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy_fin(MsLevel a, int64_t h,
                                                        uint64_t* __restrict__ gacc) {
   __shared__ uint64_t red[kWavesPerBlock];
   if (!ms_active(a) || ms_mode(a, a.L) != kModePull) return;
-  uint64_t disc = 0, edges = 0;
+  uint64_t disc = 0, edges = 0, lanes[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) lanes[k] = 0;
   for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < h;
        v += (int64_t)gridDim.x * kT) {
     uint64_t s0[K], f[K], x[K];
@@ -839,6 +886,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy_fin(MsLevel a, int64_t h,
       a.fn[v * K + k] = nxt;
       if ((s | nxt) != s0[k]) a.seen[v * K + k] = s | nxt;
       disc += (uint64_t)__popcll(nxt);
+      lanes[k] |= nxt;
       nz = nz || nxt != 0;
     }
     if (nz) {
@@ -846,6 +894,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy_fin(MsLevel a, int64_t h,
       edges += a.deg[v];
     }
   }
+  ms_note_lanes<K>(a, lanes);
   ms_flush(a, red, disc, edges);
 }
 
@@ -1138,6 +1187,8 @@ MsLevel level_args(const g500_msbfs* b, int L) {
   a.fbn = b->fb[(L + 1) & 1].get();
   a.cnt = b->cnt.get();
   a.fe = b->fe.get();
+  a.act_l = b->act.get() + (int64_t)L * 4;
+  a.act_n = b->act.get() + (int64_t)(L + 1) * 4;
   a.thr = b->td_threshold;
   a.d_roots = b->roots.get();
   a.stream_frac = b->tuning.stream_frac;
@@ -1330,11 +1381,11 @@ void enqueue_batch(const g500_msbfs* b, int K, hipStream_t s) {
   hipLaunchKernelGGL(k_ms_init, dim3(igrid), dim3(kT), 0, s, b->roots.get(), g->n_c, K,
                      b->seen.get(), b->F[0]->get(), b->fb[0].get(), b->fb[1].get(),
                      bitmap_words(b), b->acc.get(), b->bin_heavy * (int64_t)K, b->cnt.get(),
-                     b->fe.get(), kMaxLevels + 2, b->te.get());
+                     b->fe.get(), kMaxLevels + 2, b->te.get(), b->act.get());
   G500_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(G500_MSBFS_MAX_ROOTS), 0, s, b->roots.get(),
                      g->orig2new.get(), g->deg_new.get(), g->n_c, K, b->seen.get(),
-                     b->F[0]->get(), b->fb[0].get(), b->cnt.get(), b->fe.get());
+                     b->F[0]->get(), b->fb[0].get(), b->cnt.get(), b->fe.get(), b->act.get());
   G500_CHECK_LAUNCH();
   for (int L = 0; L + 1 < cap; ++L) enqueue_level(b, K, L, s);
   enqueue_output(b, K, s);
@@ -1464,6 +1515,7 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
   b->fb[0].alloc((size_t)bitmap_words(b));
   b->fb[1].alloc((size_t)bitmap_words(b));
   b->fe.alloc(kMaxLevels + 2);
+  b->act.alloc((size_t)(kMaxLevels + 2) * 4);
   b->cnt0.alloc((size_t)g->n);
   G500_CHECK_HIP(hipHostMalloc((void**)&b->h_roots, (G500_MSBFS_MAX_ROOTS + 1) * sizeof(int64_t),
                                hipHostMallocDefault));

@@ -42,6 +42,7 @@ struct g500_msbfs {
 
   g500::DevBuf<uint64_t> fb[2];             // frontier bitmaps (level & 1), n_c bits
   g500::DevBuf<unsigned long long> fe;      // frontier degree sum per level [cap + 2]
+  g500::DevBuf<uint64_t> act;               // lanes present in each level's frontier [(cap + 2) * 4]
 
   g500_msbfs_tuning tuning{};
   int64_t bin_heavy = 0, bin_medium = 0;    // new-ID bin ends for the tuning

@@ -99,6 +99,32 @@ def test_scale16_256_roots_equal_plain_engine(G, golden):
     assert (te == giant).sum() >= 250  # roots in the giant component
 
 
+@pytest.mark.parametrize("dense_algo", [1, 2])
+def test_batch_with_roots_in_tiny_components(G, dense_algo):
+    """Roots on isolated edges finish after one level while the others sweep
+    the giant component: their lanes leave the frontier, and the bottom-up
+    kernels count absent lanes as covered.  Every root's levels must still
+    equal the oracle's, in the pull, streaming and hybrid dense forms."""
+    print(f"\n[msbfs] scale=16 tiny-component roots dense_algo={dense_algo}", flush=True)
+    g = G.Graph.generate(16).construct()
+    o0, o1 = O.generate_edges(16)
+    og = O.Graph(o0, o1)
+    a = og.arrays()
+    n_c = og.info.n_c
+    deg = (a["row_off"][1:n_c + 1] - a["row_off"][:n_c]).astype(np.int64)
+    lone = [int(a["new2orig"][v]) for v in range(n_c)
+            if deg[v] == 1 and deg[int(a["col"][a["row_off"][v]])] == 1][:3]
+    assert len(lone) == 3, "SCALE 16 has six isolated edges"
+    roots = np.array(list(g.find_roots(125)) + lone, np.int64)
+    mb = G.MultiBfs(g)
+    mb.set_tuning(dense_algo=dense_algo, stream_frac=0.2 if dense_algo == 1 else 0.0)
+    mb.run(roots)
+    for i, r in enumerate(roots.tolist()):
+        ref = og.bfs(r)
+        assert np.array_equal(mb.levels(i), np.where(ref < 0, 0xFF, ref).astype(np.uint8)), (i, r)
+    mb.destroy()
+
+
 @pytest.mark.parametrize("tuning", [
     dict(heavy_degree=1, medium_degree=1, medium_group=8),          # all one wave each
     dict(heavy_degree=1 << 40, medium_degree=1 << 40, medium_group=8),  # all one lane each
