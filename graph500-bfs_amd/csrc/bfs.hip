@@ -28,10 +28,15 @@
 //                 written once per bitmap word either way.
 //   k_queue_count, k_queue_write   bitmap -> queue at the bottom-up ->
 //                 top-down switch and after top-down levels run as sweeps.
-//   k_pred_gather after the last level: pred[o] for every original ID o,
-//                 gathered from the entries the levels wrote by new ID
-//                 (levels write densely by new ID; one coalesced pass then
-//                 produces pred in original-ID order).
+//   k_pred_gather_core / k_pred_gather   pred[o] by original ID, gathered
+//                 from the entries the levels wrote by new ID (levels write
+//                 densely by new ID; one pass in original-ID order then
+//                 produces pred).  Without CoreBFS it walks only the
+//                 vertices with edges and starts on a second stream at the
+//                 bottom-up -> top-down switch, against a bitmap snapshot;
+//                 k_pred_fixup adds the vertices the last levels reach.
+//   k_level_report, k_queue_report   one block per level: the level's
+//                 totals to the host.
 //
 // Level hand-off: every block of a level's kernels writes its totals to a
 // per-block slot; a one-block report kernel queued behind them sums the
