@@ -11,7 +11,7 @@ For the batched engine a run is the span from one k_ms_init to the next;
 reports the median time per level and kernel.
 
 usage:
-  python rocprof_summary.py times  KT_KERNEL_TRACE.csv
+  python rocprof_summary.py times  KT_KERNEL_TRACE.csv [LAST_N]
   python rocprof_summary.py traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_BYTES
   python rocprof_summary.py batched KT_KERNEL_TRACE.csv GROUPS
   python rocprof_summary.py batched-traffic FETCH.csv WRITE.csv FETCH_SCALE WRITE_SCALE GROUPS
@@ -27,7 +27,8 @@ def short(name):
 
 
 def _end_of_run(names):
-    return max(i for i, n in enumerate(names) if "k_pred_gather" in n or "k_write_path" in n)
+    return max(i for i, n in enumerate(names)
+               if "k_pred_gather" in n or "k_pred_fixup" in n or "k_write_path" in n)
 
 
 def runs(rows, name_of):
@@ -40,7 +41,7 @@ def runs(rows, name_of):
     return out
 
 
-def times(path):
+def times(path, last=0):
     rows = list(csv.DictReader(open(path)))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     spans, per = [], collections.defaultdict(list)
@@ -53,6 +54,10 @@ def times(path):
             per[k].append(c.get(k, 0.0))
     print(f"BFS runs: {len(spans)}; first-kernel-start to last-kernel-end per run: "
           f"mean {statistics.mean(spans):.1f} us, median {statistics.median(spans):.1f} us")
+    if last:
+        tail = spans[-last:]
+        print(f"the last {len(tail)} runs (the timed ones): mean {statistics.mean(tail):.1f} us, "
+              f"median {statistics.median(tail):.1f} us")
     print(f"kernel-busy time per run: mean {sum(statistics.mean(v) for v in per.values()):.1f} us")
     for k, v in sorted(per.items(), key=lambda kv: -statistics.mean(kv[1])):
         print(f"  {k:26s} {statistics.mean(v):9.1f} us/run")
@@ -145,7 +150,7 @@ def batched_traffic(fetch, write, fscale, wscale, groups):
 
 if __name__ == "__main__":
     if sys.argv[1] == "times":
-        times(sys.argv[2])
+        times(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
     elif sys.argv[1] == "batched":
         batched(sys.argv[2], sys.argv[3])
     elif sys.argv[1] == "batched-traffic":
