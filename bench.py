@@ -1,20 +1,31 @@
 """Graph500 BFS benchmark on one MI355X: harmonic-mean GTEPS at SCALE 26.
 
-A step is one direction-optimizing BFS from one root of the Kronecker
-SCALE-26 graph (edgefactor 16, seed (2,3)), as the reference times
-`run_bfs` (mpi/main.cc:618-626).  Roots come from the reference's root rule
-(find_roots + redraw when a root reaches <= 3 levels, mpi/main.cc:334-359).
-The graph is generated and built on the GPU before timing; pred stays on the
-device during the timed BFS (get_pred is outside the timer, as in the
-reference).  Every timed root is validated afterwards on the device and its
-traversed-edge count (TE) is the TEPS numerator.
+The benchmark unit is the reference's: one direction-optimizing BFS per root
+over a set of 64 roots of the Kronecker SCALE-26 graph (edgefactor 16, seed
+(2,3)), each run timed around `run_bfs` (mpi/main.cc:610-676).  A step is
+one pass over the whole root set (64 BFS runs), so `--steps K` times K x 64
+runs and the harmonic mean covers every one of them.  Roots come from the
+reference's root rule (find_roots + redraw when a root reaches <= 3 levels,
+mpi/main.cc:334-359).  The graph is generated and built on the GPU before
+timing; pred stays on the device during the timed BFS (get_pred is outside
+the timer, as in the reference).  Every run of the first timed pass is
+validated on the device right after it, outside its timer, on that run's own
+pred (reference -v 2); its traversed-edge count (TE) is the TEPS numerator.
+Later passes must reproduce the first pass's frontier sizes level by level.
 
-Two secondary measurements ride on the same line: "batched" (the
-bit-parallel multi-source BFS on the same graph, 64/128/256 roots per launch,
-traversed edges/s and its ratio to the plain form) and "serving" (continuous
-batching of BFS queries over four resident graphs, SCALE 20/23/25/27, open
-Poisson arrivals; p50/p99 latency and queries/s beside a one-query-per-launch
-baseline).
+The roofline object describes the dominant kernel (the one with the most
+device time per BFS in the timed runs, from HIP events on the stream it runs
+on): its algorithmic bytes per launch -- counted on the device in an untimed
+counting pass over the same roots -- over its mean launch time.  A second
+figure does the same for the whole BFS, and the top-down yardstick
+4 m_c + 8 (n_c+1) + 12 N of the north star is reported under its own name.
+
+Secondary measurements on the same line: "scale24" (the SCALE-24 drop-in
+case, 64 validated roots, with the device time between levels), "batched"
+(the bit-parallel multi-source BFS, 64/128/256 roots per launch, traversed
+edges/s and its ratio to the plain form) and "serving" (continuous batching
+over four resident graphs, SCALE 20/23/25/27, open Poisson arrivals; p50/p99
+latency and queries/s beside a one-query-per-launch baseline).
 
 Prints one JSON line.
 """
@@ -35,8 +46,9 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="timed BFS runs (one root each)")
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=5,
+                    help="timed passes over the root set (one BFS per root each)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes before them")
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--roots", type=int, default=64)
@@ -50,6 +62,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched multi-source BFS measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-scale24", action="store_true", help="skip the SCALE-24 drop-in leg")
     ap.add_argument("--batched-no-tune", action="store_true",
                     help="batched leg with the default tuning (no autotuning; for profiling)")
     ap.add_argument("--serve-queries", type=int, default=4000)
@@ -66,16 +79,19 @@ def log(msg):
 PF_NEDGE = None
 
 
-def profiled_traffic(args):
-    """HBM bytes per BFS from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this same workload (None for any other workload)."""
+def profiled_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this same workload (None for any other
+    workload or kernel)."""
     if (args.scale, args.edgefactor, args.alpha, args.beta) != (26, 16, 10.0, 14.0):
         return None
     path = os.path.join(ROOT, "profiles", "bfs_scale26_hbm_traffic.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_bfs:"):
-                return int(line.split(":")[1])
+            f = line.split()
+            # "  <kernel>  fetch <MB> MB  write <MB> MB  per launch"
+            if len(f) >= 7 and f[0] == kernel and f[1] == "fetch" and f[4] == "write":
+                return int((float(f[2]) + float(f[5])) * 1e6)
     except OSError:
         pass
     return None
@@ -99,9 +115,10 @@ def cpu_threads():
 
 
 def cpu_baseline(graph, roots, te_by_root, budget_s, args):
-    """Oracle direction-optimizing BFS (C port, OpenMP over the host threads,
-    as the reference's CPU path runs) on the same graph and roots, until the
-    time budget is spent."""
+    """The reference's CPU path restated (the oracle's C direction-optimizing
+    BFS with OpenMP over the host threads, as the reference splits its
+    top-down frontier and bottom-up vertex range over threads) on the same
+    graph and roots, in order, until the time budget is spent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as C
     import oracle_lib as O
@@ -132,11 +149,12 @@ def cpu_baseline(graph, roots, te_by_root, budget_s, args):
             break
     O.lib().orc_graph_free(og)
     hm = len(times) / sum(t / e for t, e in zip(times, tes))
-    return {"value": hm / 1e9, "unit": "GTEPS", "cores": nt, "kind": "port",
-            "sample": f"{len(times)} root(s) of the same SCALE {args.scale} graph (the bench's "
-                      f"roots, in order), C direction-optimizing BFS with OpenMP over {nt} "
-                      f"host threads (oracle/g500_oracle.c orc_do_bfs_omp), "
-                      f"{sum(times):.1f} s"}
+    return {"value": round(hm / 1e9, 3), "unit": "GTEPS", "cores": nt, "kind": "port",
+            "label": "restated: the reference's CPU path is not on this host",
+            "sample": f"{len(times)} of the {len(roots)} timed roots of the same SCALE "
+                      f"{args.scale} graph, in order, direction-optimizing BFS in C with OpenMP "
+                      f"over {nt} host threads (oracle/g500_oracle.c orc_do_bfs_omp), "
+                      f"{sum(times):.1f} s; TE per root from the GPU run's validation"}
 
 
 def batched(g500bfs, graph, bfs, args, plain_teps):
@@ -240,11 +258,82 @@ def serving(g500bfs, args):
     return res
 
 
+def select_roots(graph, bfs, args):
+    """Roots by the reference's rule, redrawn while any reaches <= 3 levels
+    (mpi/main.cc:334-359)."""
+    r1 = 0
+    while True:
+        roots = graph.find_roots(args.roots, r1=r1)
+        ok = True
+        for r in roots:
+            at = bfs.run(int(r), args.edgefactor, args.alpha, args.beta)
+            if args.scale > 6 and at[8] <= 3:
+                ok = False
+        if ok:
+            return roots
+        r1 += 1
+        log("picking roots again")
+
+
+def frontier_trace(bfs):
+    return tuple((t["top_down"], t["nq_size"]) for t in bfs.stats()[1])
+
+
+def scale24_leg(g500bfs, args):
+    """The SCALE-24 drop-in case: 64 roots by the reference rule, each timed
+    run validated on its own pred; then one untimed pass with level timing for
+    the device time between levels."""
+    scale = 24
+    graph = g500bfs.Graph.generate(scale, args.edgefactor, 2, 3).construct()
+    bfs = g500bfs.Bfs(graph)
+    sub = argparse.Namespace(**vars(args))
+    sub.scale = scale
+    roots = select_roots(graph, bfs, sub)
+    golden = pf_nedge(scale) if args.edgefactor == 16 else None
+    times, gpu, tes, valid = [], [], [], True
+    for r in roots:
+        r = int(r)
+        t0 = time.perf_counter()
+        bfs.run(r, args.edgefactor, args.alpha, args.beta)
+        times.append(time.perf_counter() - t0)
+        gpu.append(bfs.stats(1)[0].gpu_ms)
+        ok, te, _ = bfs.validate(r)
+        valid &= ok and (golden is None or te == golden)
+        tes.append(te)
+    times, gpu, tes = np.array(times), np.array(gpu), np.array(tes, float)
+    bfs.set_timing(True)
+    gaps, nb, levels = 0.0, 0, 0
+    for r in roots:
+        bfs.run(int(r), args.edgefactor, args.alpha, args.beta)
+        st = bfs.stats(1)[0]
+        gaps += st.gap_ms
+        nb += st.gaps
+        levels += st.nlevels
+    bfs.set_timing(False)
+    out = {"workload": "plain single-root BFS, Kronecker SCALE 24, edgefactor 16, 64 roots",
+           "value_gteps": round(len(times) / np.sum(times / tes) / 1e9, 3),
+           "gpu_event_gteps": round(len(gpu) / np.sum(gpu * 1e-3 / tes) / 1e9, 3),
+           "median_bfs_ms": round(float(np.median(times)) * 1e3, 4),
+           "all_roots_valid": bool(valid), "te_per_root": int(tes[0]),
+           "levels_per_bfs": round(levels / len(roots), 2),
+           "between_levels_us": round(gaps / max(nb, 1) * 1e3, 2),
+           "between_levels_note": "device time from the end of one level's kernels to the "
+                                  "start of the next level's (host decision and launch; at the "
+                                  "bottom-up -> top-down switch also the frontier conversion)"}
+    log(f"scale24: {out}")
+    bfs.destroy()
+    graph.free()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built.", flush=True)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", flush=True)
         sys.exit(2)
     import g500bfs
     g500bfs.load()
@@ -260,56 +349,92 @@ def main():
         f"gen {t_gen:.2f}s construct {t_con:.2f}s dev {inf.device_bytes / 2**30:.1f} GiB")
 
     bfs = g500bfs.Bfs(graph)
-    # root selection with the reference's redraw rule (mpi/main.cc:334-359)
-    r1 = 0
-    while True:
-        roots = graph.find_roots(args.roots, r1=r1)
-        levels_ok = True
+    roots = [int(r) for r in select_roots(graph, bfs, args)]
+    nr = len(roots)
+    for _ in range(args.warmup):
         for r in roots:
-            at = bfs.run(int(r), args.edgefactor, args.alpha, args.beta)
-            if args.scale > 6 and at[8] <= 3:
-                levels_ok = False
-        if levels_ok:
-            break
-        r1 += 1
-        log("picking roots again")
-    for i in range(args.warmup):
-        bfs.run(int(roots[i % len(roots)]), args.edgefactor, args.alpha, args.beta)
+            bfs.run(r, args.edgefactor, args.alpha, args.beta)
 
-    # timed region: exactly `steps` BFS runs
-    nsteps = args.steps
-    times = np.zeros(nsteps)
-    gpu_ms = np.zeros(nsteps)
-    t_all = time.perf_counter()
-    for i in range(nsteps):
-        r = int(roots[i % len(roots)])
+    # timed: `steps` passes over the root set, one host timer per run
+    # (mpi/main.cc:618-626); the first pass validates each run's own pred
+    # right after it, outside its timer
+    golden = pf_nedge(args.scale) if args.edgefactor == 16 else None
+    runs = args.steps * nr
+    times = np.zeros(runs)
+    gpu_ms = np.zeros(runs)
+    gather_ms = np.zeros(runs)
+    sweep_ms = np.zeros(runs)
+    sweeps = np.zeros(runs, np.int64)
+    te_by_root, trace_by_root = {}, {}
+    all_valid, traces_repeat = True, True
+    for k in range(runs):
+        r = roots[k % nr]
         t0 = time.perf_counter()
         bfs.run(r, args.edgefactor, args.alpha, args.beta)
-        times[i] = time.perf_counter() - t0
-        gpu_ms[i] = bfs.stats(1)[0].gpu_ms
-    t_all = time.perf_counter() - t_all
+        times[k] = time.perf_counter() - t0
+        st, _ = bfs.stats(1)
+        gpu_ms[k], gather_ms[k], sweep_ms[k], sweeps[k] = (st.gpu_ms, st.gather_ms,
+                                                           st.sweep_ms, st.sweeps)
+        if k < nr:
+            trace_by_root[r] = frontier_trace(bfs)
+            if args.no_validate:
+                te_by_root[r] = golden
+            else:
+                ok, te, errs = bfs.validate(r)
+                all_valid &= ok
+                te_by_root[r] = te
+                if not ok:
+                    log(f"root {r}: validation failed {errs}")
+                if golden is not None and te != golden:
+                    log(f"root {r}: TE {te} != pf_nedge[{args.scale}] {golden}")
+        elif frontier_trace(bfs) != trace_by_root[r]:
+            traces_repeat = False
+            log(f"root {r}: frontier sizes differ from the first pass")
+    tes = np.array([te_by_root[roots[k % nr]] for k in range(runs)], float)
+    teps_hm = runs / np.sum(times / tes)
+    gpu_teps_hm = runs / np.sum(gpu_ms * 1e-3 / tes)
 
-    # untimed: validate each distinct timed root (reference -v 2 semantics)
-    te_by_root = {}
-    golden = pf_nedge(args.scale) if args.edgefactor == 16 else None
-    all_valid = True
-    for i in range(min(nsteps, len(roots))):
-        r = int(roots[i])
-        if args.no_validate:
-            te_by_root[r] = golden
-            continue
+    # untimed counting pass over the same roots: the bytes each run needs
+    bfs.set_counting(True)
+    alg, alg_sweep, alg_gather = {}, {}, {}
+    for r in roots:
         bfs.run(r, args.edgefactor, args.alpha, args.beta)
-        ok, te, errs = bfs.validate(r)
-        all_valid &= ok
-        te_by_root[r] = te
-        if golden is not None and te != golden:
-            log(f"root {r}: TE {te} != pf_nedge[{args.scale}] {golden}")
-    tes = np.array([te_by_root[int(roots[i % len(roots)])] for i in range(nsteps)], float)
-    teps_hm = nsteps / np.sum(times / tes)
-    gpu_teps_hm = nsteps / np.sum(gpu_ms * 1e-3 / tes)
+        st, _ = bfs.stats(1)
+        if frontier_trace(bfs) != trace_by_root[r]:
+            traces_repeat = False
+        alg[r], alg_sweep[r], alg_gather[r] = (st.alg_bytes, st.sweep_alg_bytes,
+                                               st.gather_alg_bytes)
+    bfs.set_counting(False)
+    alg_runs = np.array([alg[roots[k % nr]] for k in range(runs)], float)
+    sweep_b = np.array([alg_sweep[roots[k % nr]] for k in range(runs)], float)
+    gather_b = np.array([alg_gather[roots[k % nr]] for k in range(runs)], float)
+    mean_gpu_s = float(np.mean(gpu_ms)) * 1e-3
 
-    # roofline yardstick for one root's BFS (same component for every root)
-    r0 = int(roots[0])
+    # dominant kernel: most device time per BFS over the timed runs
+    kern = {
+        "k_pred_gather_core": dict(ms=gather_ms, launches=np.ones(runs), bytes=gather_b,
+                                   per_unit="24 B per vertex with edges (its original ID, "
+                                            "new ID, entry read and written) + the visited "
+                                            "bitmap"),
+        "k_bu_sweep": dict(ms=sweep_ms, launches=sweeps.astype(float), bytes=sweep_b,
+                           per_unit="per launch: old, multi and new bitmap (24 B per 64 "
+                                    "vertices), an 8-B first-neighbour pair per unvisited "
+                                    "vertex, 8 B per vertex found, 4 B per list entry, and "
+                                    "the deferred rows' offsets and entries up to the first "
+                                    "visited neighbour (counted on the device)"),
+    }
+    dom = max(kern, key=lambda k: float(np.sum(kern[k]["ms"])))
+    d = kern[dom]
+    nl = float(np.sum(d["launches"]))
+    launch_ms = float(np.sum(d["ms"])) / nl
+    launch_bytes = float(np.sum(d["bytes"])) / nl
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = profiled_traffic(args, dom)
+    bfs_gbs = float(np.mean(alg_runs)) / mean_gpu_s / 1e9
+
+    # the north star's top-down yardstick for one root's BFS (same component
+    # for every root)
+    r0 = roots[0]
     bfs.run(r0, args.edgefactor, args.alpha, args.beta)
     if args.levels:
         bfs.set_timing(True)
@@ -326,12 +451,8 @@ def main():
     nn = csr["orig2new"][reached].astype(np.int64)
     m_c = int((csr["row_off"][nn + 1] - csr["row_off"][nn]).sum())
     n_comp = int(reached.sum())
-    b_alg = 4 * m_c + 8 * (n_comp + 1) + 12 * inf.n
+    b_yard = 4 * m_c + 8 * (n_comp + 1) + 12 * inf.n
     te0 = te_by_root[r0]
-    b_per_te = b_alg / te0
-    mean_gpu_s = float(np.mean(gpu_ms)) * 1e-3
-    achieved_gbs = b_alg / mean_gpu_s / 1e9
-    traffic = profiled_traffic(args)
     del csr
 
     cpu = None
@@ -339,13 +460,16 @@ def main():
         try:
             cpu = cpu_baseline(graph, roots, te_by_root, args.cpu_seconds, args)
         except Exception as e:  # the baseline is a report, never the product
-            log(f"cpu baseline failed: {e}")
+            log(f"cpu baseline failed: {e!r}")
 
     bat = None
     if not args.no_batched:
         bat = batched(g500bfs, graph, bfs, args, teps_hm)
     bfs.destroy()
     graph.free()
+    s24 = None
+    if not args.no_scale24 and args.scale != 24:
+        s24 = scale24_leg(g500bfs, args)
     srv = None
     if not args.no_serve:
         srv = serving(g500bfs, args)
@@ -355,9 +479,9 @@ def main():
         "value": round(teps_hm / 1e9, 3),
         "unit": "GTEPS",
         "n_gpus": 1,
-        "steps": nsteps,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_all / nsteps * 1e3, 4),
+        "ms_per_step": round(float(np.sum(times)) / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -365,34 +489,51 @@ def main():
         "data": "synthetic (Graph500 Kronecker generator, seed (2,3), generated on device)",
         "config": {
             "workload": f"plain single-root direction-optimizing BFS, Kronecker SCALE {args.scale}, "
-                        f"edgefactor {args.edgefactor}, {len(roots)} roots",
-            "scale": args.scale, "edgefactor": args.edgefactor, "roots": len(roots),
+                        f"edgefactor {args.edgefactor}, {nr} roots; a step is one BFS per root",
+            "scale": args.scale, "edgefactor": args.edgefactor, "roots": nr,
+            "roots_timed": nr, "bfs_runs_timed": runs,
             "alpha": args.alpha, "beta": args.beta,
             "n": inf.n, "n_c": inf.n_c, "m": inf.m,
             "component_vertices": n_comp, "component_edges": m_c,
             "te_per_root": te0, "all_roots_valid": bool(all_valid),
+            "validated": "every run of the first timed pass, on its own pred"
+                         if not args.no_validate else "none",
+            "later_passes_repeat_frontiers": bool(traces_repeat),
             "gpu_event_hm_gteps": round(gpu_teps_hm / 1e9, 3),
             "median_bfs_ms": round(float(np.median(times)) * 1e3, 4),
+            "mean_bfs_gpu_ms": round(mean_gpu_s * 1e3, 4),
             "generation_s": round(t_gen, 3), "construction_s": round(t_con, 3),
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": round(achieved_gbs, 1),
+            "kernel": dom,
+            "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_unit": "bytes per BFS (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
-            "kernel": "one root's BFS (all level kernels on the engine stream, HIP events)",
-            "bytes_per_te": round(b_per_te, 3),
-            "mean_bfs_gpu_ms": round(mean_gpu_s * 1e3, 4),
-            "measured_gbps": (round(traffic / mean_gpu_s / 1e9, 1) if traffic else None),
-            "note": ("achieved = the fixed top-down yardstick B_alg = 4 m_c + 8 (n_c+1) + 12 N "
-                     "bytes per root over the mean BFS time; the bottom-up levels skip most "
-                     "edges, so frac can pass 1. measured_gbps = measured HBM traffic per BFS "
-                     "over the same time"),
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                            "profiles/bfs_scale26_hbm_traffic.md)",
+            "alg_bytes_per_launch": int(launch_bytes),
+            "alg_bytes_per_unit": d["per_unit"],
+            "launch_ms": round(launch_ms, 4),
+            "launches_per_bfs": round(nl / runs, 3),
+            "kernel_ms_per_bfs": {k: round(float(np.sum(v["ms"])) / runs, 4)
+                                  for k, v in kern.items()},
+            "bfs": {"alg_bytes": int(np.mean(alg_runs)),
+                    "achieved": round(bfs_gbs, 1),
+                    "frac": round(bfs_gbs / HBM_PEAK_GBS, 4),
+                    "bytes_per_te": round(float(np.mean(alg_runs)) / te0, 3),
+                    "note": "bytes every kernel of one BFS needs (counted on the device) "
+                            "over the mean BFS device time"},
+            "yardstick": {"bytes": int(b_yard), "bytes_per_te": round(b_yard / te0, 3),
+                          "ratio": round(b_yard / mean_gpu_s / 1e9 / HBM_PEAK_GBS, 4),
+                          "note": "4 m_c + 8 (n_c+1) + 12 N: every component edge read once, "
+                                  "as top-down would; the bottom-up levels skip most of them, "
+                                  "so this ratio is not bounded by 1"},
         },
         "cpu_baseline": cpu,
+        "scale24": s24,
         "batched": bat,
         "serving": srv,
     }

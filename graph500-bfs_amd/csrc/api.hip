@@ -2,6 +2,7 @@
 //
 // Each entry point sets the handle's device, runs the C++ implementation and
 // turns exceptions into status codes plus a per-thread message.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -91,6 +92,50 @@ bool has_edge_host(g500_graph* g, int64_t v) {
   return g500::has_parent_host(g, v);
 }
 
+// Edges [0, count) of (v0, v1) to the device after the graph's current ones;
+// every endpoint must fit 32 bits.  Copies through a bounded host staging
+// buffer, so a caller's chunk of any size costs at most 8 Mi packed edges of
+// host memory.
+void append_edges(g500_graph* g, const int64_t* v0, const int64_t* v1, int64_t count) {
+  if (count < 0 || (count > 0 && (!v0 || !v1))) g500::fail(G500_ERR_ARG, "bad edge arrays");
+  if (g->constructed) g500::fail(G500_ERR_STATE, "graph already constructed");
+  if (count > g->capacity - g->nedges)
+    g500::fail(G500_ERR_ARG, "%lld more edges exceed the capacity of %lld (%lld held)",
+               (long long)count, (long long)g->capacity, (long long)g->nedges);
+  constexpr int64_t kStage = int64_t(1) << 23;
+  std::vector<uint2> packed((size_t)std::min(count, kStage));
+  for (int64_t b0 = 0; b0 < count; b0 += kStage) {
+    const int64_t nb = std::min(kStage, count - b0);
+    uint64_t ors = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+      const int64_t a = v0[b0 + i], c = v1[b0 + i];
+      if (a < 0 || c < 0 || a > 0xFFFFFFFFll || c > 0xFFFFFFFFll)
+        g500::fail(G500_ERR_ARG, "edge %lld has an endpoint outside [0, 2^32)",
+                   (long long)(g->nedges + i));
+      packed[(size_t)i] = make_uint2((uint32_t)a, (uint32_t)c);
+      ors |= (uint64_t)(a | c);
+    }
+    G500_CHECK_HIP(hipMemcpy(g->edges.get() + g->nedges, packed.data(), nb * sizeof(uint2),
+                             hipMemcpyHostToDevice));
+    g->nedges += nb;
+    g->id_or |= ors;
+  }
+  g->id_bits = g->id_or ? 64 - __builtin_clzll(g->id_or) : 0;
+}
+
+g500_graph* new_graph_with_capacity(int device, int64_t capacity) {
+  if (capacity < 0) g500::fail(G500_ERR_ARG, "negative edge capacity");
+  g500_graph* g = new_graph(device);
+  try {
+    g->edges.alloc((size_t)capacity);
+    g->capacity = capacity;
+  } catch (...) {
+    delete g;
+    throw;
+  }
+  return g;
+}
+
 }  // namespace
 
 g500_graph::~g500_graph() {
@@ -115,6 +160,23 @@ int g500_graph_generate(int device, int scale, int edgefactor, uint64_t seed1,
   });
 }
 
+int g500_graph_create(int device, int64_t capacity, g500_graph** out) {
+  return guard([&] {
+    if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    *out = new_graph_with_capacity(device, capacity);
+  });
+}
+
+int g500_graph_append_edges(g500_graph* g, const int64_t* v0, const int64_t* v1,
+                            int64_t count) {
+  return guard([&] {
+    need_graph(g, false);
+    use_device(g->device);
+    append_edges(g, v0, v1, count);
+  });
+}
+
 int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
                           int64_t nedges, g500_graph** out) {
   return guard([&] {
@@ -122,23 +184,9 @@ int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
     *out = nullptr;
     if (nedges < 0 || (nedges > 0 && (!v0 || !v1)))
       g500::fail(G500_ERR_ARG, "bad edge arrays");
-    std::vector<uint2> packed((size_t)nedges);
-    uint64_t maxid = 0;
-    for (int64_t i = 0; i < nedges; ++i) {
-      if (v0[i] < 0 || v1[i] < 0 || v0[i] > 0xFFFFFFFFll || v1[i] > 0xFFFFFFFFll)
-        g500::fail(G500_ERR_ARG, "edge %lld has an endpoint outside [0, 2^32)",
-                   (long long)i);
-      packed[(size_t)i] = make_uint2((uint32_t)v0[i], (uint32_t)v1[i]);
-      maxid |= (uint64_t)(v0[i] | v1[i]);
-    }
-    g500_graph* g = new_graph(device);
+    g500_graph* g = new_graph_with_capacity(device, nedges);
     try {
-      g->nedges = nedges;
-      g->id_bits = maxid ? 64 - __builtin_clzll(maxid) : 0;
-      g->edges.alloc((size_t)nedges);
-      if (nedges)
-        G500_CHECK_HIP(hipMemcpy(g->edges.get(), packed.data(), nedges * sizeof(uint2),
-                                 hipMemcpyHostToDevice));
+      append_edges(g, v0, v1, nedges);
     } catch (...) {
       delete g;
       throw;
@@ -301,6 +349,11 @@ int g500_make_random_numbers(int64_t nvalues, uint64_t seed1, uint64_t seed2,
 int g500_graph_free(g500_graph* g) {
   return guard([&] {
     if (!g) return;
+    // an engine keeps pointers into the graph (and reads g->device when it
+    // is destroyed): freeing the graph first would leave them dangling
+    if (g->live_engines > 0)
+      g500::fail(G500_ERR_STATE, "graph still has %d engine(s): destroy them first",
+                 g->live_engines);
     (void)hipSetDevice(g->device);
     (void)hipStreamSynchronize(g->stream);
     delete g;
@@ -308,20 +361,28 @@ int g500_graph_free(g500_graph* g) {
 }
 
 int g500_bfs_create(g500_graph* g, g500_bfs** out) {
+  return g500_bfs_create_ex(g, 16, 10.0, 14.0, out);
+}
+
+int g500_bfs_create_ex(g500_graph* g, int edgefactor, double alpha, double beta,
+                       g500_bfs** out) {
   return guard([&] {
     need_graph(g, true);
     if (!out) g500::fail(G500_ERR_ARG, "null output pointer");
     *out = nullptr;
+    if (!(alpha > 0) || !(beta > 0) || edgefactor <= 0)
+      g500::fail(G500_ERR_ARG, "alpha, beta and edgefactor must be positive");
     use_device(g->device);
     auto* b = new g500_bfs;
     try {
       b->g = g;
       G500_CHECK_HIP(hipStreamCreate(&b->stream));
-      g500::bfs_init(b);
+      g500::bfs_init(b, edgefactor, alpha, beta);
     } catch (...) {
       delete b;
       throw;
     }
+    ++g->live_engines;
     *out = b;
   });
 }
@@ -379,6 +440,13 @@ int g500_bfs_set_exec_alpha(g500_bfs* b, double exec_alpha) {
   });
 }
 
+int g500_bfs_set_counting(g500_bfs* b, int on) {
+  return guard([&] {
+    if (!b) g500::fail(G500_ERR_ARG, "null bfs handle");
+    b->counting = on != 0;
+  });
+}
+
 int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st, g500_level_trace* levels,
                        int max_levels) {
   return guard([&] {
@@ -394,7 +462,12 @@ int g500_bfs_destroy(g500_bfs* b) {
   return guard([&] {
     if (!b) return;
     (void)hipSetDevice(b->g->device);
+    // a run that failed inside a level can leave kernels queued on either
+    // stream (the pred gather runs on gstream): drain both before the
+    // buffers and the pinned report they write are released
     (void)hipStreamSynchronize(b->stream);
+    if (b->gstream) (void)hipStreamSynchronize(b->gstream);
+    --b->g->live_engines;
     delete b;
   });
 }
@@ -444,6 +517,7 @@ int g500_msbfs_create_ex(g500_graph* g, int max_roots, int flags, g500_msbfs** o
       delete b;
       throw;
     }
+    ++g->live_engines;
     *out = b;
   });
 }
@@ -563,6 +637,7 @@ int g500_msbfs_destroy(g500_msbfs* b) {
     if (!b) return;
     (void)hipSetDevice(b->g->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    --b->g->live_engines;
     delete b;
   });
 }

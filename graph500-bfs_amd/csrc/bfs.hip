@@ -511,13 +511,17 @@ __device__ inline uint64_t readlane64(uint64_t x, int l) {
 
 // Rows with more than max_rest entries left are not scanned: long[j] is set
 // for them instead.
+// With `count`, *scan_bytes grows by the bytes the scans need: the row's
+// offsets (16 B), each entry read up to the first visited one (4 B) and the
+// parent's original ID (4 B).
 __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[kBuIlp],
                                bool (&hit)[kBuIlp], bool (&lng)[kBuIlp],
                                int64_t max_rest, const VisProbe& visited,
                                const int64_t* __restrict__ row_off,
                                const uint32_t* __restrict__ col,
                                const uint32_t* __restrict__ n2o,
-                               int64_t* __restrict__ pred_new, uint64_t level_bits) {
+                               int64_t* __restrict__ pred_new, uint64_t level_bits,
+                               bool count, uint64_t* scan_bytes) {
   int64_t k[kBuIlp], k1[kBuIlp];
 #pragma unroll
   for (int j = 0; j < kBuIlp; ++j) {
@@ -525,6 +529,7 @@ __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[k
     k1[j] = scan[j] ? row_off[v[j] + 1] : 0;
     lng[j] = k1[j] - k[j] > max_rest;
     if (lng[j]) k1[j] = k[j];
+    if (count && scan[j]) *scan_bytes += 16;
   }
   uint32_t par[kBuIlp];
 #pragma unroll
@@ -538,12 +543,15 @@ __device__ inline void bu_rows(const uint32_t (&v)[kBuIlp], const bool (&scan)[k
       bool h[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) h[t] = (k[j] + t < k1[j]) && visited(u[t]);
+      int used = (int)min<int64_t>(4, k1[j] - k[j]);
 #pragma unroll
       for (int t = 3; t >= 0; --t)
         if (h[t]) {
           hit[j] = true;
           par[j] = u[t];
+          used = t + 1;
         }
+      if (count) *scan_bytes += 4u * (uint64_t)used + (hit[j] ? 4u : 0u);
       k[j] = hit[j] ? k1[j] : k[j] + 4;
     }
   }
@@ -589,14 +597,15 @@ __global__ __launch_bounds__(kBuT) void k_bu_sweep(
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
     uint32_t* __restrict__ unv_cnt, uint32_t* __restrict__ longv, LevelSync* ls,
-    uint64_t* __restrict__ cpart, uint64_t level_bits) {
+    uint64_t* __restrict__ cpart, uint64_t level_bits, uint64_t* __restrict__ wacc) {
   __shared__ unsigned long long fw[kBuWaves][kWave];
   __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t nchunks = ceil_div(words, kWave);
-  uint64_t nfound = 0;
+  const bool count = wacc != nullptr;  // counting runs tally the row-scan bytes
+  uint64_t nfound = 0, scan_b = 0;
   for (int64_t c = (int64_t)blockIdx.x * kBuWaves + wave; c < nchunks;
        c += (int64_t)gridDim.x * kBuWaves) {
     const int64_t w = c * kWave + lane;
@@ -698,6 +707,8 @@ __global__ __launch_bounds__(kBuT) void k_bu_sweep(
           dk = dlo & ~(int64_t)3;
           have = true;
           lng = dk1 - dlo > kBuLongRow;
+          // row offsets, and the deferred entry written and read back
+          if (count) scan_b += 24;
         }
         if (__ballot(have) == 0) break;
         const uint64_t lm = __ballot(lng);
@@ -723,12 +734,19 @@ __global__ __launch_bounds__(kBuT) void k_bu_sweep(
 #pragma unroll
           for (int t = 0; t < kBuStep; ++t)
             h[t] = dk + t >= dlo && dk + t < dk1 && visited(u[t]);
+          int stop = (int)min<int64_t>(kBuStep, dk1 - dk);
 #pragma unroll
           for (int t = kBuStep - 1; t >= 0; --t)
             if (h[t]) {
               hit = true;
               par = u[t];
+              stop = t + 1;
             }
+          if (count) {
+            // entries of the row read in this step, up to the first visited one
+            const int start = dlo > dk ? (int)(dlo - dk) : 0;
+            scan_b += 4u * (uint64_t)max(stop - start, 0) + (hit ? 4u : 0u);
+          }
           dk += kBuStep;
           miss = !hit && dk >= dk1;
         }
@@ -747,10 +765,17 @@ __global__ __launch_bounds__(kBuT) void k_bu_sweep(
     if (w < words) vis_new[w] = myword | newbits;
     nfound += (uint64_t)__popcll(newbits);
     if (lane == 0) unv_cnt[c] = wp;
+    // the entries of the vertices found, and the chunk's unvisited list
+    if (count) scan_b += 8u * (uint64_t)__popcll(newbits) + (lane == 0 ? 4u * (uint64_t)wp : 0u);
     wave_sync();
   }
   const uint64_t s = block_sum<kBuWaves>(nfound, red);
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
+  if (count) {
+    __syncthreads();  // thread 0 has read red
+    const uint64_t sb = block_sum<kBuWaves>(scan_b, red);
+    if (threadIdx.x == 0) wacc[blockIdx.x] += sb;  // the block's own slot
+  }
 }
 
 // The rows k_bu_sweep listed, a wave each: 256 entries per step, first
@@ -762,20 +787,23 @@ __global__ __launch_bounds__(kBuT) void k_bu_long(
     const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
     uint32_t* __restrict__ unv, uint32_t* unv_cnt,
     const uint32_t* __restrict__ longv, const LevelSync* ls,
-    uint64_t* __restrict__ cpart, uint64_t level_bits) {
+    uint64_t* __restrict__ cpart, uint64_t level_bits, uint64_t* __restrict__ wacc) {
   __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t n_long = (int64_t)ls->cnt[2];
-  uint64_t nfound = 0;
+  const bool count = wacc != nullptr;
+  uint64_t nfound = 0, scan_b = 0;
   for (int64_t it = (int64_t)blockIdx.x * kBuWaves + wave; it < n_long;
        it += (int64_t)gridDim.x * kBuWaves) {
     const uint32_t v = longv[it];
-    int64_t k = row_off[v] + 1;
+    const int64_t k0 = row_off[v] + 1;
+    int64_t k = k0;
     const int64_t k1 = row_off[v + 1];
     bool found = false;
     uint32_t par = 0;
+    int64_t at = k1;  // entry holding the first visited neighbour
     while (!found && k < k1) {
       uint32_t u[4];
       bool h[4];
@@ -792,10 +820,15 @@ __global__ __launch_bounds__(kBuT) void k_bu_long(
         if (m && !found) {
           found = true;
           par = __shfl(u[i], __ffsll((long long)m) - 1, kWave);
+          at = k + i * kWave + (__ffsll((long long)m) - 1);
         }
       }
       k += 4 * kWave;
     }
+    // row offsets, the listed entry, entries up to the first visited one,
+    // then the parent's original ID or the unvisited-list entry
+    if (count && lane == 0)
+      scan_b += 20 + 4 * (uint64_t)((found ? at + 1 : k1) - k0) + (found ? 12 : 4);
     if (lane == 0) {
       if (found) {
         put_pred(pred_new + v, (uint64_t)n2o[par] | level_bits);
@@ -810,6 +843,11 @@ __global__ __launch_bounds__(kBuT) void k_bu_long(
   }
   const uint64_t s = block_sum<kBuWaves>(nfound, red);
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
+  if (count) {
+    __syncthreads();
+    const uint64_t sb = block_sum<kBuWaves>(scan_b, red);
+    if (threadIdx.x == 0) wacc[blockIdx.x] += sb;
+  }
 }
 
 // Four waves per SIMD (128 VGPRs, a 12-byte spill) rather than the three its
@@ -821,14 +859,16 @@ __global__ __launch_bounds__(kBuT) __attribute__((amdgpu_waves_per_eu(4))) void 
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
-    uint32_t* __restrict__ unv_cnt, uint64_t* __restrict__ cpart, uint64_t level_bits) {
+    uint32_t* __restrict__ unv_cnt, uint64_t* __restrict__ cpart, uint64_t level_bits,
+    uint64_t* __restrict__ wacc) {
   __shared__ unsigned long long fw[kBuWaves][kWave];
   __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int64_t nchunks = ceil_div(words, kWave);
-  uint64_t nfound = 0;
+  const bool count = wacc != nullptr;
+  uint64_t nfound = 0, scan_b = 0;
   for (int64_t c = (int64_t)blockIdx.x * kBuWaves + wave; c < nchunks;
        c += (int64_t)gridDim.x * kBuWaves) {
     const int64_t w = c * kWave + lane;
@@ -882,7 +922,8 @@ __global__ __launch_bounds__(kBuT) __attribute__((amdgpu_waves_per_eu(4))) void 
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) scan[j] = ok[j] && !hit[j] && multi_bit(multi, v[j]);
       bool lng[kBuIlp];
-      bu_rows(v, scan, hit2, lng, INT64_MAX, visited, row_off, col, n2o, pred_new, level_bits);
+      bu_rows(v, scan, hit2, lng, INT64_MAX, visited, row_off, col, n2o, pred_new, level_bits,
+              count, &scan_b);
 #pragma unroll
       for (int j = 0; j < kBuIlp; ++j) {
         hit[j] = hit[j] || hit2[j];
@@ -896,10 +937,16 @@ __global__ __launch_bounds__(kBuT) __attribute__((amdgpu_waves_per_eu(4))) void 
     if (w < words) vis_new[w] = myword | fbits;
     nfound += (uint64_t)__popcll(fbits);
     if (lane == 0) unv_cnt[c] = wp;
+    if (count) scan_b += 8u * (uint64_t)__popcll(fbits) + (lane == 0 ? 4u * (uint64_t)wp : 0u);
     wave_sync();
   }
   const uint64_t s = block_sum<kBuWaves>(nfound, red);
   if (threadIdx.x == 0) cpart[blockIdx.x] = s;
+  if (count) {
+    __syncthreads();
+    const uint64_t sb = block_sum<kBuWaves>(scan_b, red);
+    if (threadIdx.x == 0) wacc[blockIdx.x] += sb;
+  }
 }
 
 // queue <- bits set in cur but not in prev; reports (count, degree sum).
@@ -1017,6 +1064,26 @@ __global__ __launch_bounds__(kT) void k_queue_report(const uint32_t* __restrict_
 // that vertex was reached, the root's own entry, else -1.  With CoreBFS a
 // forest vertex not reached through the core gets its forest parent with
 // depth 0xFFFF (write_tree_parents, mpi/corebfs_adaptor.hpp:285-301).
+// The GPU deals blocks to its 8 XCDs in turn (block b runs on XCD b % 8),
+// and each XCD has its own L2.  The gathers read pred_new at new IDs that,
+// within one degree class, rise with the original ID (the relabel breaks ties
+// by original ID): neighbouring items share those cache lines.  So each XCD
+// takes one contiguous eighth of the items and its blocks stride through
+// that eighth together; the grid is a multiple of kXcds.
+constexpr int kXcds = 8;
+struct XcdSpan {
+  int64_t lo, hi, rank, nblk;  // this XCD's items [lo, hi); block rank among its nblk
+};
+__device__ inline XcdSpan xcd_span(int64_t n) {
+  const int64_t per = ceil_div(n, (int64_t)kXcds);
+  XcdSpan x;
+  x.lo = min((int64_t)(blockIdx.x % kXcds) * per, n);
+  x.hi = min(x.lo + per, n);
+  x.rank = blockIdx.x / kXcds;
+  x.nblk = gridDim.x / kXcds;
+  return x;
+}
+
 __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__ o2n,
                                                    const uint64_t* __restrict__ vis,
                                                    const int64_t* __restrict__ pred_new,
@@ -1024,8 +1091,8 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
                                                    const uint32_t* __restrict__ tree_parent,
                                                    const uint64_t* __restrict__ has_parent,
                                                    int64_t* __restrict__ pred) {
-  for (int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x; o < n;
-       o += (int64_t)gridDim.x * kT) {
+  const XcdSpan xs = xcd_span(n);
+  for (int64_t o = xs.lo + xs.rank * kT + threadIdx.x; o < xs.hi; o += xs.nblk * kT) {
     const uint32_t v = o2n[o];
     int64_t e = -1;
     if ((int64_t)v < n_c && ((vis[v >> 6] >> (v & 63)) & 1ull)) {
@@ -1041,18 +1108,18 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
 
 // Without CoreBFS, a vertex without edges is never reached and its pred
 // entry is always -1: the engine fills pred once and the gather walks only
-// core_o, the original IDs of the vertices with edges in ascending order.
-__global__ __launch_bounds__(kT) void k_pred_gather_core(const uint32_t* __restrict__ core_o,
+// core, the (original ID, new ID) pairs of the vertices with edges in
+// ascending original-ID order.
+__global__ __launch_bounds__(kT) void k_pred_gather_core(const uint2* __restrict__ core,
                                                         int64_t n_core,
-                                                        const uint32_t* __restrict__ o2n,
                                                         const uint64_t* __restrict__ vis,
                                                         const int64_t* __restrict__ pred_new,
                                                         int64_t root,
                                                         int64_t* __restrict__ pred) {
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n_core;
-       i += (int64_t)gridDim.x * kT) {
-    const uint32_t o = core_o[i];
-    const uint32_t v = o2n[o];
+  const XcdSpan xs = xcd_span(n_core);
+  for (int64_t i = xs.lo + xs.rank * kT + threadIdx.x; i < xs.hi; i += xs.nblk * kT) {
+    const uint2 ov = get_pair(core + i);
+    const uint32_t o = ov.x, v = ov.y;
     int64_t e = ((vis[v >> 6] >> (v & 63)) & 1ull) ? pred_new[v] : -1;
     if ((int64_t)o == root) e = root;
     __builtin_nontemporal_store(e, pred + o);
@@ -1066,10 +1133,11 @@ __global__ __launch_bounds__(kT) void k_core_flags(const uint32_t* __restrict__ 
 }
 
 __global__ __launch_bounds__(kT) void k_core_list(const uint32_t* __restrict__ flag,
-                                                 const uint64_t* __restrict__ off, int64_t n,
-                                                 uint32_t* __restrict__ core_o) {
+                                                 const uint64_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ o2n, int64_t n,
+                                                 uint2* __restrict__ core) {
   const int64_t o = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (o < n && flag[o]) core_o[off[o]] = (uint32_t)o;
+  if (o < n && flag[o]) core[off[o]] = make_uint2((uint32_t)o, o2n[o]);
 }
 
 // pred entries of the vertices reached after the snapshot the early gather
@@ -1118,7 +1186,7 @@ unsigned grid_for(int64_t work_items, int64_t per_block) {
 
 }  // namespace
 
-void bfs_init(g500_bfs* b) {
+void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   g500_graph* g = b->g;
   b->words = ceil_div(std::max<int64_t>(g->n_c, 1), 64);
   b->pred.alloc((size_t)g->n);
@@ -1153,6 +1221,7 @@ void bfs_init(g500_bfs* b) {
                              ceil_div(chunks, kBuWaves)));
   b->lpart.alloc((size_t)std::max<int64_t>(
       kMaxGrid, std::max<int64_t>(b->bu_grid[0] + b->bu_grid[2], b->bu_grid[1])));
+  b->wacc.alloc((size_t)b->bu_grid[0] + b->bu_grid[2] + b->bu_grid[1]);
   b->lsync.alloc(1);
   G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
   G500_CHECK_HIP(hipHostMalloc((void**)&b->h_report, sizeof(HostReport),
@@ -1163,6 +1232,8 @@ void bfs_init(g500_bfs* b) {
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_snap, hipEventDisableTiming));
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
+  G500_CHECK_HIP(hipEventCreate(&b->ev_g0));
+  G500_CHECK_HIP(hipEventCreate(&b->ev_g1));
   G500_CHECK_HIP(hipStreamCreate(&b->gstream));
   b->vis_snap.alloc((size_t)b->words);
   b->qcnt.alloc((size_t)kQueueGrid * kWavesPerBlock);
@@ -1180,9 +1251,9 @@ void bfs_init(g500_bfs* b) {
     G500_CHECK_LAUNCH();
     ScanWorkspace ws;
     exclusive_scan<uint32_t>(flag.get(), off.get(), g->n, ws, s);
-    b->core_o.alloc((size_t)g->n_c);
+    b->core.alloc((size_t)g->n_c);
     hipLaunchKernelGGL(k_core_list, dim3((unsigned)ceil_div(g->n, kT)), dim3(kT), 0, s,
-                       flag.get(), off.get(), g->n, b->core_o.get());
+                       flag.get(), off.get(), g->orig2new.get(), g->n, b->core.get());
     G500_CHECK_LAUNCH();
     uint64_t total = 0;
     G500_CHECK_HIP(hipMemcpyAsync(&total, off.get() + g->n, sizeof(total),
@@ -1200,7 +1271,7 @@ void bfs_init(g500_bfs* b) {
     // corebfs_index::reset_unreachable: one BFS over the core from its
     // highest-degree vertex decides which forest parents survive
     if (g->hd_root >= 0) {
-      bfs_run(b, g->hd_root, 16, 10.0, 14.0, nullptr);
+      bfs_run(b, g->hd_root, edgefactor, alpha, beta, nullptr);
       corebfs_reset_unreachable(g, b->vis.get(), b->stream);
     } else {
       corebfs_reset_unreachable(g, nullptr, b->stream);
@@ -1240,17 +1311,21 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
 // reads need every wave the chip holds).
 void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t st) {
   g500_graph* g = b->g;
+  G500_CHECK_HIP(hipEventRecord(b->ev_g0, st));
+  // grids are a multiple of kXcds (xcd_span)
+  auto grid = [](int64_t items) {
+    return (unsigned)std::min<int64_t>(ceil_div(ceil_div(items, kT), kXcds) * kXcds, 65536);
+  };
   if (b->n_core > 0)
-    hipLaunchKernelGGL(k_pred_gather_core,
-                       dim3((unsigned)std::min<int64_t>(ceil_div(b->n_core, kT), 65536)),
-                       dim3(kT), 0, st, b->core_o.get(), b->n_core, g->orig2new.get(), vis,
-                       b->pred_new.get(), root, b->pred.get());
+    hipLaunchKernelGGL(k_pred_gather_core, dim3(grid(b->n_core)), dim3(kT), 0, st,
+                       b->core.get(), b->n_core, vis, b->pred_new.get(), root, b->pred.get());
   else
-    hipLaunchKernelGGL(k_pred_gather, dim3((unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536)),
+    hipLaunchKernelGGL(k_pred_gather, dim3(grid(g->n)),
                        dim3(kT), 0, st, g->orig2new.get(), vis, b->pred_new.get(), g->n, g->n_c,
                        root, g->corebfs ? g->tree_parent.get() : nullptr,
                        g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
   G500_CHECK_LAUNCH();
+  G500_CHECK_HIP(hipEventRecord(b->ev_g1, st));
 }
 
 // The sweep lists its long rows in nq, which no bottom-up level uses (a
@@ -1260,25 +1335,37 @@ void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_b
   g500_graph* g = b->g;
   hipStream_t s = b->stream;
   int parts = 0;
+  // counting runs: per-block scan bytes, sweep / long-row / list slots
+  uint64_t* wsweep = b->counting ? b->wacc.get() : nullptr;
+  uint64_t* wlong = b->counting ? b->wacc.get() + b->bu_grid[0] : nullptr;
+  uint64_t* wlist = b->counting ? b->wacc.get() + b->bu_grid[0] + b->bu_grid[2] : nullptr;
   if (lists) {
     hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kBuT), 0, s, b->vis.get(),
                        b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
                        g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
-                       b->unv.get(), b->unv_cnt.get(), b->lpart.get(), level_bits);
+                       b->unv.get(), b->unv_cnt.get(), b->lpart.get(), level_bits, wlist);
     G500_CHECK_LAUNCH();
     parts = (int)b->bu_grid[1];
   } else {
+    const size_t k = 2 * (size_t)b->nsweeps++;
+    while (b->sweep_events.size() < k + 2) {
+      hipEvent_t e;
+      G500_CHECK_HIP(hipEventCreate(&e));
+      b->sweep_events.push_back(e);
+    }
+    G500_CHECK_HIP(hipEventRecord(b->sweep_events[k], s));
     hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kBuT), 0, s, b->vis.get(),
                        b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
                        g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
                        b->unv.get(), b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
-                       b->lpart.get(), level_bits);
+                       b->lpart.get(), level_bits, wsweep);
     G500_CHECK_LAUNCH();
+    G500_CHECK_HIP(hipEventRecord(b->sweep_events[k + 1], s));
     hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kBuT), 0, s, b->vis.get(),
                        (unsigned long long*)b->vis_alt.get(), b->words, g->row_off.get(),
                        g->col.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
                        b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
-                       b->lpart.get() + b->bu_grid[0], level_bits);
+                       b->lpart.get() + b->bu_grid[0], level_bits, wlong);
     G500_CHECK_LAUNCH();
     parts = (int)(b->bu_grid[0] + b->bu_grid[2]);
   }
@@ -1354,6 +1441,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->trace.clear();
   b->stats = g500_run_stats{};
   b->have_result = false;
+  b->nsweeps = 0;
   // a gather left running by a run that failed must not overlap this one
   G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
   if (b->n_core > 0 && b->pred_fill_needed) {
@@ -1363,6 +1451,22 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     b->pred_fill_needed = false;
     b->iso_root = -1;
   }
+  // counting runs: the bytes each kernel needs (closed form per level from
+  // the level totals, plus the row-scan bytes the bottom-up kernels count)
+  int64_t alg = 0, sweep_host = 0, root_deg = 0;
+  if (b->counting) {
+    G500_CHECK_HIP(hipMemsetAsync(b->wacc.get(), 0, b->wacc.n * sizeof(uint64_t), s));
+    if (root_in) {
+      int64_t ro[2];
+      G500_CHECK_HIP(hipMemcpyAsync(ro, g->row_off.get() + root_new, sizeof(ro),
+                                    hipMemcpyDeviceToHost, s));
+      G500_CHECK_HIP(hipStreamSynchronize(s));
+      root_deg = ro[1] - ro[0];
+    }
+  }
+  int64_t visited = root_in ? 1 : 0;  // vertices with edges reached so far
+  int64_t after_snap = 0;             // of them, reached after the gather's snapshot
+  alg += 8 * words;                   // init writes the bitmap
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1,
@@ -1407,6 +1511,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     const bool exec_bu =
         !td || (b->exec_alpha > 0 && est_edges > 0 &&
                 (double)est_edges > (double)g->m / b->exec_alpha);
+    const bool lists_before = unv_ok;  // a bottom-up level walks the lists
     if (td && exec_bu) {
       launch_bu(b, false, seq, level_bits, false);
       unv_ok = true;
@@ -1450,6 +1555,29 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     tr.nq_edges = nq_edges;
     tr.exec_bottom_up = exec_bu ? 1 : 0;
     b->trace.push_back(tr);
+    {
+      // bytes this level needs beyond the bottom-up kernels' counted scans
+      const int64_t unvisited = n_c - visited;
+      if ((td && exec_bu) || (!td && !lists_before)) {
+        // sweep: old bitmap, multi bitmap and new bitmap; a first-neighbour
+        // pair per unvisited vertex
+        const int64_t sw = 24 * words + 8 * unvisited;
+        sweep_host += sw;
+        alg += sw;
+        if (td) alg += 16 * words + 8 * nq_size;  // bitmap -> queue (queue + degree)
+      } else if (!td) {
+        // list level: old and new bitmap; a list entry and a pair per listed vertex
+        alg += 16 * words + 12 * unvisited;
+      } else {
+        // top-down: queue entry, row offsets and original ID per frontier
+        // vertex, one column entry per frontier edge, and per vertex found its
+        // entry, queue slot and degree (plus the prefix sums of a large frontier)
+        const int64_t fe = est_edges < 0 ? root_deg : est_edges;
+        alg += 24 * qlen + 4 * fe + 16 * nq_size + (qlen > kTdWin ? 16 * qlen : 0);
+      }
+      visited += nq_size;
+      if (early_gather) after_snap += nq_size;
+    }
     if (nq_size == 0) break;
 
     // direction decision (mpi/bfs.hpp:3413-3457)
@@ -1488,6 +1616,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       // vertices the last levels reach are patched in at the end
       // (k_pred_fixup)
       const unsigned long long cseq = ++b->seq;
+      alg += 24 * words + 8 * nq_size;  // bitmap -> queue with the snapshot
       launch_to_queue(b, b->q.get(), cseq, b->vis_snap.get());
       G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
       launch_gather(b, root, b->vis_snap.get(), b->gstream);
@@ -1508,9 +1637,17 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                        b->vis.get(), b->vis_snap.get(), words, g->new2orig.get(),
                        b->pred_new.get(), b->pred.get());
     G500_CHECK_LAUNCH();
+    // both bitmaps; per vertex its new -> original ID, entry read and written
+    alg += 16 * words + 20 * after_snap;
   } else {
     launch_gather(b, root, b->vis.get(), s);
   }
+  // the gather: per vertex with edges its original ID, new ID, entry read and
+  // written, plus the bitmap; with CoreBFS every original ID (new ID and
+  // entry written) and the reached vertices' entries
+  const int64_t gather_bytes = b->n_core > 0 ? 24 * b->n_core + 8 * words
+                                             : 12 * g->n + 8 * visited + 8 * words;
+  alg += gather_bytes;
   if (b->n_core > 0) {
     // the gather skips vertices without edges: a root among them gets its
     // own entry here, and the last such root's entry goes back to -1
@@ -1537,6 +1674,24 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_begin, b->ev_end));
   b->stats.gpu_ms = ms;
   b->stats.nlevels = (int)b->trace.size();
+  G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_g0, b->ev_g1));
+  b->stats.gather_ms = ms;
+  for (int i = 0; i < b->nsweeps; ++i) {
+    G500_CHECK_HIP(hipEventElapsedTime(&ms, b->sweep_events[2 * i], b->sweep_events[2 * i + 1]));
+    b->stats.sweep_ms += ms;
+  }
+  b->stats.sweeps = b->nsweeps;
+  if (b->counting) {
+    std::vector<uint64_t> w(b->wacc.n);
+    G500_CHECK_HIP(hipMemcpy(w.data(), b->wacc.get(), w.size() * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost));
+    uint64_t sweep_dev = 0, other_dev = 0;
+    for (size_t i = 0; i < w.size(); ++i) (i < b->bu_grid[0] ? sweep_dev : other_dev) += w[i];
+    b->stats.counted = 1;
+    b->stats.alg_bytes = alg + (int64_t)(sweep_dev + other_dev);
+    b->stats.sweep_alg_bytes = b->nsweeps ? sweep_host + (int64_t)sweep_dev : 0;
+    b->stats.gather_alg_bytes = gather_bytes;
+  }
   if (b->timing) {
     for (size_t i = 0; i < b->trace.size(); ++i) {
       float lm = 0;
@@ -1544,6 +1699,12 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                                          b->level_events[2 * i + 1]));
       b->trace[i].gpu_ms = lm;
       (b->trace[i].top_down ? b->stats.td_ms : b->stats.bu_ms) += lm;
+      if (i > 0) {
+        G500_CHECK_HIP(hipEventElapsedTime(&lm, b->level_events[2 * i - 1],
+                                           b->level_events[2 * i]));
+        b->stats.gap_ms += lm;
+        ++b->stats.gaps;
+      }
     }
   }
   for (const auto& tr : b->trace)
@@ -1573,6 +1734,9 @@ g500_bfs::~g500_bfs() {
   if (ev_end) (void)hipEventDestroy(ev_end);
   if (ev_snap) (void)hipEventDestroy(ev_snap);
   if (ev_gather) (void)hipEventDestroy(ev_gather);
+  if (ev_g0) (void)hipEventDestroy(ev_g0);
+  if (ev_g1) (void)hipEventDestroy(ev_g1);
+  for (auto e : sweep_events) (void)hipEventDestroy(e);
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto e : level_events) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);

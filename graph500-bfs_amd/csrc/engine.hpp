@@ -55,14 +55,23 @@ struct g500_bfs {
   g500::DevBuf<uint64_t> lpart;   // per-block totals of a level's kernels
   g500::DevBuf<uint32_t> qcnt;    // per-wave counts of the bitmap -> queue conversion
   g500::DevBuf<uint64_t> qedges;  // its per-block degree sums
-  // without CoreBFS: original IDs of the vertices with edges, ascending (the
-  // gather's domain; every other pred entry stays -1)
-  g500::DevBuf<uint32_t> core_o;
+  // without CoreBFS: (original ID, new ID) of the vertices with edges, in
+  // ascending original ID (the gather's domain; every other pred entry stays -1)
+  g500::DevBuf<uint2> core;
   int64_t n_core = 0;
   int64_t iso_root = -1;  // a root without edges whose entry the last run wrote
   bool pred_fill_needed = false;  // pred must be reset to -1 before the next run
   std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
   bool timing = false;
+  // per-kernel timing (always on): the gather's events on the stream it runs
+  // on, and a pair around each k_bu_sweep launch of a run
+  hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;
+  std::vector<hipEvent_t> sweep_events;
+  int nsweeps = 0;
+  // counting runs: per-block row-scan bytes of the bottom-up kernels, one
+  // slot per block (sweep, long-row and list kernels in that order)
+  bool counting = false;
+  g500::DevBuf<uint64_t> wacc;
   double exec_alpha = 15.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
 
   bool have_result = false;
@@ -74,7 +83,9 @@ struct g500_bfs {
 };
 
 namespace g500 {
-void bfs_init(g500_bfs* b);
+// allocates the engine; with CoreBFS the first engine of a graph also runs
+// the reachability BFS with these settings (mpi/corebfs_adaptor.hpp:228-249)
+void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta);
 void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
              double beta, int64_t* at_data);
 void bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,

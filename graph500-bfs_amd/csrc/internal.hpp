@@ -15,6 +15,8 @@ struct g500_graph {
   // input edge list (original IDs), kept resident for validation
   g500::DevBuf<uint2> edges;
   int64_t nedges = 0;
+  int64_t capacity = 0;  // edges the buffer holds (appended graphs)
+  uint64_t id_or = 0;    // OR of every input ID appended so far
   int id_bits = 0;  // every input ID < 2^id_bits
   bool symmetric_input = false;  // edges already hold both directions
 
@@ -52,6 +54,10 @@ struct g500_graph {
 
   g500::ScanWorkspace scan_ws;
   g500::RadixWorkspace radix_ws;
+
+  // engines (plain and batched) created on this graph and not yet destroyed:
+  // they hold pointers into it, so the graph may only be freed after them
+  int live_engines = 0;
 
   int64_t device_bytes() const;
   ~g500_graph();

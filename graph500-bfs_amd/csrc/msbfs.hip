@@ -1691,28 +1691,41 @@ void msbfs_wait(g500_msbfs* b) {
   b->have_result = true;
 }
 
-void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels) {
-  if (count <= 0) return;
+namespace {
+
+void check_pairs(const g500_msbfs* b, const int64_t* pairs, int64_t count) {
   for (int64_t i = 0; i < count; ++i)
     if (pairs[2 * i] < 0 || pairs[2 * i] >= b->nroots)
       fail(G500_ERR_ARG, "pair %lld: root index %lld outside [0, %d)", (long long)i,
            (long long)pairs[2 * i], b->nroots);
-  if ((int64_t)b->gather_pairs.n < 2 * count) {
-    b->gather_pairs.alloc((size_t)2 * count);
-    b->gather_levels.alloc((size_t)count);
-  }
+}
+
+// Pairs to d_pairs, one thread per pair, levels back from d_levels.
+void gather_staged(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels,
+                   int64_t* d_pairs, uint8_t* d_levels) {
   const g500_graph* g = b->g;
   hipStream_t s = b->stream;
-  G500_CHECK_HIP(hipMemcpyAsync(b->gather_pairs.get(), pairs, (size_t)count * 2 * sizeof(int64_t),
+  G500_CHECK_HIP(hipMemcpyAsync(d_pairs, pairs, (size_t)count * 2 * sizeof(int64_t),
                                 hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_ms_gather, dim3((unsigned)ceil_div(count, kT)), dim3(kT), 0, s,
                      b->roots.get(), g->orig2new.get(), g->n, g->n_c, b->K,
                      (const uint64_t* const*)b->fptr.get(), b->cnt.get(), b->levels_cap(),
-                     b->gather_pairs.get(), count, b->gather_levels.get());
+                     d_pairs, count, d_levels);
   G500_CHECK_LAUNCH();
-  G500_CHECK_HIP(hipMemcpyAsync(levels, b->gather_levels.get(), (size_t)count,
-                                hipMemcpyDeviceToHost, s));
+  G500_CHECK_HIP(hipMemcpyAsync(levels, d_levels, (size_t)count, hipMemcpyDeviceToHost, s));
   G500_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels) {
+  if (count <= 0) return;
+  check_pairs(b, pairs, count);
+  if ((int64_t)b->gather_pairs.n < 2 * count) {
+    b->gather_pairs.alloc((size_t)2 * count);
+    b->gather_levels.alloc((size_t)count);
+  }
+  gather_staged(b, pairs, count, levels, b->gather_pairs.get(), b->gather_levels.get());
 }
 
 // Coordinate search over the work split and the direction rule: each

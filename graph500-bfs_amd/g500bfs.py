@@ -5,6 +5,7 @@ the HIP library; there is no CPU fallback.  A missing library raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -42,7 +43,12 @@ class LevelTrace(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("gpu_ms", C.c_double), ("td_ms", C.c_double),
-                ("bu_ms", C.c_double), ("td_edges", C.c_int64), ("root_level", C.c_int64)]
+                ("bu_ms", C.c_double), ("td_edges", C.c_int64), ("root_level", C.c_int64),
+                ("gather_ms", C.c_double), ("sweep_ms", C.c_double),
+                ("sweeps", C.c_int32), ("counted", C.c_int32),
+                ("alg_bytes", C.c_int64), ("sweep_alg_bytes", C.c_int64),
+                ("gather_alg_bytes", C.c_int64), ("gap_ms", C.c_double),
+                ("gaps", C.c_int32), ("reserved", C.c_int32)]
 
 
 class MsbfsTuning(C.Structure):
@@ -76,6 +82,8 @@ def load():
     sig = {
         "g500_graph_generate": [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.POINTER(vp)],
         "g500_graph_from_edges": [C.c_int, i64p, i64p, i64, C.POINTER(vp)],
+        "g500_graph_create": [C.c_int, i64, C.POINTER(vp)],
+        "g500_graph_append_edges": [vp, i64p, i64p, i64],
         "g500_graph_construct": [vp, C.c_int],
         "g500_graph_get_info": [vp, C.POINTER(GraphInfo)],
         "g500_graph_has_edge": [vp, i64, C.POINTER(C.c_int)],
@@ -86,12 +94,14 @@ def load():
         "g500_graph_free": [vp],
         "g500_make_random_numbers": [i64, C.c_uint64, C.c_uint64, i64, C.POINTER(C.c_double)],
         "g500_bfs_create": [vp, C.POINTER(vp)],
+        "g500_bfs_create_ex": [vp, C.c_int, C.c_double, C.c_double, C.POINTER(vp)],
         "g500_bfs_run": [vp, i64, C.c_int, C.c_double, C.c_double, i64p],
         "g500_bfs_get_pred": [vp, i64p],
         "g500_bfs_get_levels": [vp, C.POINTER(C.c_uint16)],
         "g500_bfs_validate": [vp, i64, i64p, i64p, i64p, C.POINTER(C.c_int)],
         "g500_bfs_set_timing": [vp, C.c_int],
         "g500_bfs_set_exec_alpha": [vp, C.c_double],
+        "g500_bfs_set_counting": [vp, C.c_int],
         "g500_bfs_get_stats": [vp, C.POINTER(RunStats), C.POINTER(LevelTrace), C.c_int],
         "g500_bfs_destroy": [vp],
         "g500_msbfs_create": [vp, C.c_int, C.POINTER(vp)],
@@ -161,6 +171,9 @@ class Graph:
     def __init__(self, handle):
         self.h = handle
         self._info = None
+        # engines built on this graph hold pointers into it: free() destroys
+        # the live ones first (the library refuses to free a graph under them)
+        self._engines = weakref.WeakSet()
 
     @classmethod
     def generate(cls, scale, edgefactor=16, seed1=2, seed2=3, device=0):
@@ -176,6 +189,21 @@ class Graph:
         _check(load().g500_graph_from_edges(device, _p(v0, C.c_int64), _p(v1, C.c_int64),
                                             len(v0), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def create(cls, capacity, device=0):
+        """An empty graph filled by append_edges() before construct()."""
+        h = C.c_void_p()
+        _check(load().g500_graph_create(device, int(capacity), C.byref(h)))
+        return cls(h)
+
+    def append_edges(self, v0, v1):
+        v0 = np.ascontiguousarray(v0, np.int64)
+        v1 = np.ascontiguousarray(v1, np.int64)
+        _check(load().g500_graph_append_edges(self.h, _p(v0, C.c_int64), _p(v1, C.c_int64),
+                                              len(v0)))
+        self._info = None
+        return self
 
     def construct(self, corebfs=False, symmetric_input=False):
         flags = (CONSTRUCT_COREBFS if corebfs else 0) | (CONSTRUCT_SYMMETRIC_INPUT if symmetric_input else 0)
@@ -237,6 +265,8 @@ class Graph:
 
     def free(self):
         if self.h:
+            for e in list(self._engines):
+                e.destroy()
             _check(load().g500_graph_free(self.h))
             self.h = None
 
@@ -255,6 +285,7 @@ class Bfs:
         h = C.c_void_p()
         _check(load().g500_bfs_create(graph.h, C.byref(h)))
         self.h = h
+        graph._engines.add(self)
 
     def run(self, root, edgefactor=16, alpha=10.0, beta=14.0):
         at = np.full(AT_NUM, -1, np.int64)
@@ -289,6 +320,10 @@ class Bfs:
 
     def set_timing(self, on=True):
         _check(load().g500_bfs_set_timing(self.h, int(on)))
+
+    def set_counting(self, on=True):
+        """Counting runs report the bytes their kernels need (stats())."""
+        _check(load().g500_bfs_set_counting(self.h, int(on)))
 
     def stats(self, max_levels=256):
         st = RunStats()
@@ -325,6 +360,7 @@ class MultiBfs:
             _check(load().g500_msbfs_create(graph.h, int(max_roots), C.byref(h)))
         self.h = h
         self.nroots = 0
+        graph._engines.add(self)
 
     def tuning(self):
         t = MsbfsTuning()

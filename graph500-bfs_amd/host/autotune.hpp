@@ -115,10 +115,13 @@ class AutoTuner {
       tmp = is_alpha ? at[i][kAtPreGlobalNqEdges] : at[i][kAtGlobalNqSize];
       if (tmp != kAtNotDefined) l.push_back({tmp, i});
     }
-    // std::sort in the reference; stable here so ties keep root order
+    // std::sort by value only, as the reference sorts its auto_tuning_t
+    // arrays (parameters.h:280-285): over more than 16 roots introsort
+    // reorders equal values, and the same algorithm on the same sequence
+    // reorders them the same way, so ties are nudged in the reference's order
     auto by_val = [](const Item& a, const Item& b) { return a.val < b.val; };
-    std::stable_sort(s.begin(), s.end(), by_val);
-    std::stable_sort(l.begin(), l.end(), by_val);
+    std::sort(s.begin(), s.end(), by_val);
+    std::sort(l.begin(), l.end(), by_val);
 
     bool smaller = false;
     if (s.empty()) return false;

@@ -63,20 +63,28 @@ class BfsOnMI355X {
   // EdgeList follows the reference's EdgeListStorage read protocol
   // (beginRead / read / endRead, generator/graph_generator.hpp:46-485) and
   // holds the output of make_symmetry_edge_list.
+  // Each chunk goes straight to device memory sized from num_local_edges();
+  // the host holds one chunk's endpoints at a time.
   template <typename EdgeList>
   void construct(int /*scale*/, bool corebfs, EdgeList* sym_edge_list) {
+    reset_graph();
+    check(g500_graph_create(device_, sym_edge_list->num_local_edges(), &graph_.h),
+          "graph create");
     std::vector<int64_t> v0, v1;
     const int chunks = sym_edge_list->beginRead(false);
     for (int c = 0; c < chunks; ++c) {
       typename EdgeList::edge_type* buf = nullptr;
       const int n = sym_edge_list->read(&buf);
+      v0.resize((size_t)n);
+      v1.resize((size_t)n);
       for (int i = 0; i < n; ++i) {
-        v0.push_back(buf[i].v0());
-        v1.push_back(buf[i].v1());
+        v0[(size_t)i] = buf[i].v0();
+        v1[(size_t)i] = buf[i].v1();
       }
+      check(g500_graph_append_edges(graph_.h, v0.data(), v1.data(), n), "graph upload");
     }
     sym_edge_list->endRead();
-    construct_from_edges(v0.data(), v1.data(), (int64_t)v0.size(), corebfs, true);
+    finish_construct(corebfs, true);
   }
 
   // Edge list already in memory (symmetric or not).
@@ -98,11 +106,12 @@ class BfsOnMI355X {
 
   // BfsBase::prepare_bfs (mpi/bfs.hpp:141-151).
   void prepare_bfs(int validation_level, bool pre_exec, bool real_benchmark,
-                   int /*edgefactor*/, double /*alpha*/, double /*beta*/,
-                   int64_t* /*pred*/) {
+                   int edgefactor, double alpha, double beta, int64_t* /*pred*/) {
     printInformation(validation_level, pre_exec, real_benchmark);
     end_bfs();
-    check(g500_bfs_create(graph_.h, &engine_), "prepare_bfs");
+    // with CoreBFS the reachability BFS runs with these settings, as
+    // reset_unreachable does (mpi/corebfs_adaptor.hpp:228-249)
+    check(g500_bfs_create_ex(graph_.h, edgefactor, alpha, beta, &engine_), "prepare_bfs");
     if (getenv("G500_PROFILE_LEVELS")) check(g500_bfs_set_timing(engine_, 1), "timing");
   }
 

@@ -101,6 +101,21 @@ int g500_graph_generate(int device, int scale, int edgefactor, uint64_t seed1,
 int g500_graph_from_edges(int device, const int64_t* v0, const int64_t* v1,
                           int64_t nedges, g500_graph** out);
 
+/* An empty graph on `device` whose input edge list will hold up to
+ * `capacity` edges, filled chunk by chunk with g500_graph_append_edges and
+ * then built with g500_graph_construct.  This is how BfsBase::construct
+ * (mpi/bfs.hpp:116-139) hands the driver's EdgeListStorage to the device: its
+ * chunks (beginRead/read/endRead, generator/graph_generator.hpp:46-485) go
+ * straight to device memory sized from num_local_edges(), with no host copy
+ * of the whole list. */
+int g500_graph_create(int device, int64_t capacity, g500_graph** out);
+
+/* Appends count edges (original IDs, < 2^32) after those already held;
+ * fails with G500_ERR_ARG past the capacity and G500_ERR_STATE after
+ * construction. */
+int g500_graph_append_edges(g500_graph* g, const int64_t* v0, const int64_t* v1,
+                            int64_t count);
+
 /* flags of g500_graph_construct */
 enum {
   G500_CONSTRUCT_COREBFS = 1,        /* prune trees first (-C) */
@@ -149,6 +164,9 @@ int g500_find_roots(g500_graph* g, int nroots, int r1, int r2, int64_t* roots,
 int g500_make_random_numbers(int64_t nvalues, uint64_t seed1, uint64_t seed2,
                              int64_t position, double* out);
 
+/* Frees the graph.  Every engine created on it (g500_bfs, g500_msbfs) must be
+ * destroyed first: with engines still alive the call fails with
+ * G500_ERR_STATE and frees nothing. */
 int g500_graph_free(g500_graph* g);
 
 /* ---- BFS engine (L5) --------------------------------------------------- */
@@ -157,6 +175,13 @@ int g500_graph_free(g500_graph* g);
  * with CoreBFS also corebfs_index::reset_unreachable
  * (mpi/corebfs_adaptor.hpp:228-249). */
 int g500_bfs_create(g500_graph* g, g500_bfs** out);
+
+/* The same with prepare_bfs's edgefactor, alpha and beta (mpi/bfs.hpp:141-151),
+ * which the CoreBFS reachability BFS of a graph's first engine runs with
+ * (reset_unreachable, mpi/corebfs_adaptor.hpp:228-249); g500_bfs_create uses
+ * the reference defaults 16, 10, 14. */
+int g500_bfs_create_ex(g500_graph* g, int edgefactor, double alpha, double beta,
+                       g500_bfs** out);
 
 /* BfsBase::run_bfs(root, pred, ef, alpha, beta, auto_tuning_data)
  * (mpi/bfs.hpp:3214-3232).  pred stays on the device until
@@ -195,6 +220,25 @@ typedef struct g500_run_stats {
   double td_ms, bu_ms;  /* device time in top-down / bottom-up levels */
   int64_t td_edges;     /* edges scanned by top-down levels */
   int64_t root_level;   /* level the core BFS started at (CoreBFS) */
+  /* per-kernel device time from events on the kernel's own stream */
+  double gather_ms;     /* pred gather by original ID (k_pred_gather_core / k_pred_gather) */
+  double sweep_ms;      /* bottom-up bitmap sweeps (k_bu_sweep), summed over the run */
+  int32_t sweeps;       /* k_bu_sweep launches in the run */
+  int32_t counted;      /* 1: the run counted its algorithmic bytes (below) */
+  /* Counting runs only (g500_bfs_set_counting): bytes the run's kernels need
+   * to move -- each stream read or written once, the row entries a bottom-up
+   * scan reads up to its first visited neighbour (counted on the device),
+   * visited-bitmap probes excluded -- in total and for the sweep and gather
+   * kernels (their launches in this run). */
+  int64_t alg_bytes;
+  int64_t sweep_alg_bytes;
+  int64_t gather_alg_bytes;
+  /* with level timing on (g500_bfs_set_timing): device time between the end
+   * of one level's kernels and the start of the next level's, summed over
+   * the run's level boundaries (the host's per-level decision and launch) */
+  double gap_ms;
+  int32_t gaps;         /* level boundaries summed in gap_ms */
+  int32_t reserved;
 } g500_run_stats;
 
 /* Enables per-level HIP-event timing (off by default; costs one event pair
@@ -209,6 +253,10 @@ int g500_bfs_set_timing(g500_bfs* b, int on);
  * comes from G500_EXEC_ALPHA in the environment, else 15 (on MI355X a
  * bottom-up sweep beats the top-down expansion of such levels). */
 int g500_bfs_set_exec_alpha(g500_bfs* b, double exec_alpha);
+/* Counting runs (on != 0): the bottom-up kernels also tally the bytes their
+ * row scans need and g500_bfs_get_stats reports the run's algorithmic bytes.
+ * Same results; off by default (it costs a copy after each run). */
+int g500_bfs_set_counting(g500_bfs* b, int on);
 int g500_bfs_get_stats(g500_bfs* b, g500_run_stats* st,
                        g500_level_trace* levels, int max_levels);
 

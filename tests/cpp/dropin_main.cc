@@ -16,9 +16,11 @@ struct PackedEdge {
 };
 struct MemEdgeList {
   typedef PackedEdge edge_type;
-  static const int CHUNK_SIZE = 1 << 20;
+  // small chunks: the drop-in appends each one to the device separately
+  static const int CHUNK_SIZE = 1 << 16;
   std::vector<PackedEdge> e;
   size_t pos = 0;
+  int64_t num_local_edges() const { return (int64_t)e.size(); }
   int beginRead(bool) {
     pos = 0;
     return (int)((e.size() + CHUNK_SIZE - 1) / CHUNK_SIZE);

@@ -47,6 +47,49 @@ def test_driver_validation_levels_and_skip():
     assert rc != 0 and "-s and -A" in out
 
 
+def _te_lines(out):
+    return [int(x) for x in re.findall(r"Number of traversed edges is (-?\d+)", out)]
+
+
+def test_driver_te_at_validation_level_1_with_another_edgefactor():
+    """-v 1 validates the first root and, as mpi/main.cc:645-652 does, takes
+    its count for the other roots (no pf_nedge entry exists for edgefactor 8);
+    -v 2 validates every root and finds the same count."""
+    rc, out = _run(["14", "-e", "8", "-n", "4", "-v", "1"])
+    assert rc == 0, out[-3000:]
+    te = _te_lines(out)
+    assert len(te) == 4 and te[0] > 0 and te == [te[0]] * 4
+    rc2, out2 = _run(["14", "-e", "8", "-n", "4", "-v", "2"])
+    assert rc2 == 0, out2[-3000:]
+    assert _te_lines(out2)[0] == te[0]
+    assert min(float(x) for x in re.findall(r"TEPS for BFS \d+ is (\S+)", out)) > 0
+
+
+def test_driver_scale27_corebfs_64_roots(golden):
+    """Config 3 at full size: SCALE 27 with CoreBFS pruning (-C), 64 roots by
+    the reference rule, every root validated (-v 2) with TE = pf_nedge[27]."""
+    rc, out = _run(["27", "-C", "-n", "64"], timeout=900)
+    assert rc == 0, out[-3000:]
+    b = _block(out)
+    assert b["SCALE"] == 27 and b["NBFS"] == 64
+    assert b["min_nedge"] == b["max_nedge"] == golden["pf_nedge"][27]
+    assert len(_te_lines(out)) == 64
+    print(f"SCALE 27 -C: harmonic_mean_TEPS {b['harmonic_mean_TEPS']:.4g}", flush=True)
+
+
+def test_driver_scale27_corebfs_autotune(golden):
+    """The same with the adaptive alpha/beta tuner (-A); the tuned values are
+    printed for the record, every root validated."""
+    rc, out = _run(["27", "-C", "-A", "-n", "64"], timeout=900)
+    assert rc == 0, out[-3000:]
+    b = _block(out)
+    assert b["min_nedge"] == b["max_nedge"] == golden["pf_nedge"][27]
+    est = re.findall(r"Estimated Performance = .*", out)
+    assert est
+    print(f"SCALE 27 -C -A: {est[-1]}; alpha {b['alpha']} beta {b['beta']}; "
+          f"harmonic_mean_TEPS {b['harmonic_mean_TEPS']:.4g}", flush=True)
+
+
 def test_driver_autotune():
     rc, out = _run(["16", "-n", "8", "-A"])
     assert rc == 0, out[-3000:]

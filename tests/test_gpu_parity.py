@@ -291,3 +291,43 @@ def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
                     assert t["exec_bottom_up"] == int(heavy), (r, t)
         else:
             assert all(t["exec_bottom_up"] == 1 for t in trace[1:])
+
+
+def _ref_case(scale, corebfs):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ref_autotune_traces.json")) as f:
+        return next(c for c in json.load(f)["cases"]
+                    if c["scale"] == scale and c["corebfs"] == corebfs)
+
+
+@pytest.mark.parametrize("scale,corebfs", [(16, False), (18, False), (20, False),
+                                           (16, True), (18, True)])
+def test_auto_tuning_data_matches_reference_driver_log(G, scale, corebfs, golden):
+    """Every root's auto_tuning_data against what the reference's own CPU
+    driver printed with -A (mpi/main.cc:60-78, built from its sources): roots
+    drawn by the reference rule, the T2B fields at alpha 10 and the B2T fields
+    at the alpha the reference's tuner settled on; with CoreBFS the core BFS's
+    fields.  Each run's tree validates with the reference's TE."""
+    print(f"\n[reference -A log] scale={scale} corebfs={corebfs}", flush=True)
+    case = _ref_case(scale, corebfs)
+    g = G.Graph.generate(scale).construct(corebfs=corebfs)
+    bfs = G.Bfs(g)
+    r1 = 0
+    while True:
+        roots = g.find_roots(case["nroots"], r1=r1)
+        if all(bfs.run(int(r))[8] > 3 for r in roots):
+            break
+        r1 += 1
+    tuned = float(case["tuned_alpha"])
+    for (i, t2b, nqe, m, pre), (j, b2t, nqs, nc, ef, pres) in zip(case["alpha_block"],
+                                                                 case["beta_block"]):
+        r = int(roots[i])
+        at = bfs.run(r, alpha=10.0, beta=14.0)
+        assert [at[2], at[0], at[1], at[6]] == [t2b, nqe, m, pre], (i, r)
+        at = bfs.run(r, alpha=tuned, beta=14.0)
+        assert [at[5], at[3], at[4], at[7]] == [b2t, nqs, nc, pres], (i, r)
+        ok, te, errs = bfs.validate(r)
+        assert ok and te == case["traversed_edges"][i] == golden["pf_nedge"][scale], (i, errs)
+    bfs.destroy()
+    g.free()

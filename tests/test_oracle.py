@@ -12,6 +12,7 @@ check) fails at least one of them:
   * DO schedule    -> hand-traced sequences and the logged SCALE-20 trace
 """
 import ctypes as C
+import os
 import random
 
 import numpy as np
@@ -411,3 +412,44 @@ def test_corebfs_small_forest_by_hand():
     ok, te, errs = O.validate(v0, v1, 14, g.info.n, 4, pred, True)
     assert ok, errs
     assert (pred[5] >> 48) == 4 and (pred[0] >> 48) == 3     # rebuilt depths
+
+
+# ------------------------------------- the reference driver's own -A log
+def _ref_traces():
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ref_autotune_traces.json")) as f:
+        return json.load(f)["cases"]
+
+
+def _roots_by_reference_rule(g, nroots, scale):
+    """find_roots, redrawn with the next seed while any root reaches <= 3
+    levels (mpi/main.cc:334-359)."""
+    r1 = 0
+    while True:
+        roots = g.find_roots(nroots, r1=r1)
+        if scale <= 6 or all(g.do_bfs(int(r))[1][8] > 3 for r in roots):
+            return roots
+        r1 += 1
+
+
+@pytest.mark.parametrize("scale", [16, 18])
+def test_schedule_matches_reference_driver_log(scale, golden):
+    """Per root index, the auto_tuning_data the reference's own driver printed
+    with -A (ref_runnable, built from mpi/main.cc): the T2B fields at alpha 10
+    and the B2T fields at the alpha it then settled on.  This pins the root
+    list (drawing, -ffast-math formula and redraw rule) and the direction
+    rule for every root, not only the first."""
+    case = next(c for c in _ref_traces() if c["scale"] == scale and not c["corebfs"])
+    v0, v1 = O.generate_edges(scale)
+    g = O.Graph(v0, v1)
+    roots = _roots_by_reference_rule(g, case["nroots"], scale)
+    assert len(roots) == case["nroots"]
+    tuned = float(case["tuned_alpha"])
+    for (i, t2b, nqe, m, pre), (j, b2t, nqs, nc, ef, pres) in zip(case["alpha_block"],
+                                                                 case["beta_block"]):
+        assert i == j
+        _, at, _ = g.do_bfs(int(roots[i]), ef=16, alpha=10.0, beta=14.0)
+        assert [at[2], at[0], at[1], at[6]] == [t2b, nqe, m, pre], (scale, i)
+        _, at, _ = g.do_bfs(int(roots[i]), ef=16, alpha=tuned, beta=14.0)
+        assert [at[5], at[3], at[4], at[7]] == [b2t, nqs, nc, pres], (scale, i)
+    assert case["traversed_edges"] == [golden["pf_nedge"][scale]] * case["nroots"]
