@@ -604,6 +604,16 @@ int g500_msbfs_gather_levels(g500_msbfs* b, const int64_t* pairs, int64_t count,
   });
 }
 
+int g500_msbfs_gather_levels_noalloc(g500_msbfs* b, const int64_t* pairs, int64_t count,
+                                     uint8_t* levels) {
+  return guard([&] {
+    need_result(b);
+    if (count < 0 || (count > 0 && (!pairs || !levels)))
+      g500::fail(G500_ERR_ARG, "bad pair list");
+    g500::msbfs_gather_noalloc(b, pairs, count, levels);
+  });
+}
+
 int g500_msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int rounds,
                         g500_msbfs_tuning* best, double* best_ms) {
   return guard([&] {

@@ -27,6 +27,7 @@
 // one sums per-root traversed edges.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "msbfs.hpp"
 
@@ -1483,6 +1484,39 @@ void update_bins(g500_msbfs* b) {
 
 }  // namespace
 
+// Staging of the point lookups that allocate nothing: a file-scope device
+// array that every GPU holds once for the life of the process.  Host code
+// never passes `scratch` itself to a kernel or a copy (on the host its name
+// is a stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr int64_t kNoAllocPairBytes = G500_NOALLOC_MAX_PAIRS * 2 * (int64_t)sizeof(int64_t);
+static __device__ __align__(256) unsigned char
+    scratch[kNoAllocPairBytes + G500_NOALLOC_MAX_PAIRS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+unsigned char* device_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    fail(G500_ERR_ARG, "device %d outside [0, %d)", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      fail(G500_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+           hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
+
 int64_t msbfs_footprint(const g500_graph* g, int max_roots, bool level_arrays) {
   const int64_t kmax = ceil_div(max_roots, 64);
   const int64_t nc = std::max<int64_t>(1, g->n_c);
@@ -1501,6 +1535,7 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
   g500_graph* g = b->g;
   b->max_roots = max_roots;
   b->kmax = (int)ceil_div(max_roots, 64);
+  (void)device_scratch(g->device);  // the allocation-free lookups' staging
   G500_CHECK_HIP(hipStreamCreate(&b->stream));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
@@ -1717,6 +1752,20 @@ void gather_staged(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* 
 }
 
 }  // namespace
+
+void msbfs_gather_noalloc(g500_msbfs* b, const int64_t* pairs, int64_t count,
+                          uint8_t* levels) {
+  if (count > G500_NOALLOC_MAX_PAIRS)
+    fail(G500_ERR_ARG, "%lld pairs: the allocation-free lookup takes at most %d per call",
+         (long long)count, G500_NOALLOC_MAX_PAIRS);
+  unsigned char* base = device_scratch(b->g->device);
+  if (count <= 0) return;
+  check_pairs(b, pairs, count);
+  // the kernel writes every one of the `count` level bytes it hands back,
+  // so nothing in the scratch needs clearing between calls
+  gather_staged(b, pairs, count, levels, reinterpret_cast<int64_t*>(base),
+                base + kNoAllocPairBytes);
+}
 
 void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels) {
   if (count <= 0) return;

@@ -74,6 +74,9 @@ void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots);
 bool msbfs_query(g500_msbfs* b);
 void msbfs_wait(g500_msbfs* b);
 void msbfs_gather(g500_msbfs* b, const int64_t* pairs, int64_t count, uint8_t* levels);
+// the same lookups staged in a file-scope device array (no allocation)
+void msbfs_gather_noalloc(g500_msbfs* b, const int64_t* pairs, int64_t count,
+                          uint8_t* levels);
 double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int rounds,
                       g500_msbfs_tuning* best);
 }  // namespace g500

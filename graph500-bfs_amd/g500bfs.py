@@ -65,6 +65,7 @@ class MsbfsStats(C.Structure):
 
 
 MSBFS_MAX_ROOTS = 256
+NOALLOC_MAX_PAIRS = 65536  # G500_NOALLOC_MAX_PAIRS
 
 _lib = None
 
@@ -118,6 +119,7 @@ def load():
         "g500_msbfs_device_levels": [vp, C.POINTER(vp), i64p],
         "g500_msbfs_get_te": [vp, i64p],
         "g500_msbfs_gather_levels": [vp, i64p, i64, C.POINTER(C.c_uint8)],
+        "g500_msbfs_gather_levels_noalloc": [vp, i64p, i64, C.POINTER(C.c_uint8)],
         "g500_msbfs_autotune": [vp, i64p, C.c_int, C.c_int, C.POINTER(MsbfsTuning),
                                 C.POINTER(C.c_double)],
         "g500_msbfs_get_stats": [vp, C.POINTER(MsbfsStats)],
@@ -401,15 +403,18 @@ class MultiBfs:
         _check(load().g500_msbfs_device_levels(self.h, C.byref(p), C.byref(stride)))
         return p.value, stride.value
 
-    def gather(self, root_index, vertices):
-        """Levels of `vertices` (original IDs) for batch roots `root_index`."""
+    def gather(self, root_index, vertices, noalloc=False):
+        """Levels of `vertices` (original IDs) for batch roots `root_index`.
+        noalloc=True stages them in the library's fixed device array
+        (g500_msbfs_gather_levels_noalloc, at most NOALLOC_MAX_PAIRS)."""
         ri = np.broadcast_to(np.asarray(root_index, np.int64), np.shape(vertices))
         pairs = np.ascontiguousarray(np.stack([ri, np.asarray(vertices, np.int64)], axis=-1)
                                      .reshape(-1, 2))
         out = np.empty(len(pairs), np.uint8)
-        if len(pairs):
-            _check(load().g500_msbfs_gather_levels(self.h, _p(pairs, C.c_int64), len(pairs),
-                                                   _p(out, C.c_uint8)))
+        fn = (load().g500_msbfs_gather_levels_noalloc if noalloc
+              else load().g500_msbfs_gather_levels)
+        if len(pairs) or noalloc:
+            _check(fn(self.h, _p(pairs, C.c_int64), len(pairs), _p(out, C.c_uint8)))
         return out
 
     def autotune(self, roots, rounds=1):

@@ -346,6 +346,17 @@ int g500_msbfs_device_levels(g500_msbfs* b, void** dev_ptr, int64_t* row_stride)
  * without level arrays. */
 int g500_msbfs_gather_levels(g500_msbfs* b, const int64_t* pairs, int64_t count,
                              uint8_t* levels);
+/* The same point lookups without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * pairs and levels are staged in a file-scope device array of
+ * G500_NOALLOC_MAX_PAIRS entries (about 1.1 MiB) that each GPU holds until the
+ * process ends; larger counts fail with G500_ERR_ARG.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (from several host
+ * threads) race. */
+#define G500_NOALLOC_MAX_PAIRS 65536
+int g500_msbfs_gather_levels_noalloc(g500_msbfs* b, const int64_t* pairs, int64_t count,
+                                     uint8_t* levels);
 /* Picks the tuning fields by timing batches of `roots` (coordinate search,
  * `rounds` passes of 1..8); leaves the best tuning set and returns it with its
  * device time per batch. */
