@@ -302,13 +302,14 @@ def scale24_leg(g500bfs, args):
         tes.append(te)
     times, gpu, tes = np.array(times), np.array(gpu), np.array(tes, float)
     bfs.set_timing(True)
-    gaps, nb, levels = 0.0, 0, 0
+    gaps, nb, levels, miss = 0.0, 0, 0, 0
     for r in roots:
         bfs.run(int(r), args.edgefactor, args.alpha, args.beta)
         st = bfs.stats(1)[0]
         gaps += st.gap_ms
         nb += st.gaps
         levels += st.nlevels
+        miss += st.mispredicted
     bfs.set_timing(False)
     out = {"workload": "plain single-root BFS, Kronecker SCALE 24, edgefactor 16, 64 roots",
            "value_gteps": round(len(times) / np.sum(times / tes) / 1e9, 3),
@@ -316,10 +317,14 @@ def scale24_leg(g500bfs, args):
            "median_bfs_ms": round(float(np.median(times)) * 1e3, 4),
            "all_roots_valid": bool(valid), "te_per_root": int(tes[0]),
            "levels_per_bfs": round(levels / len(roots), 2),
-           "between_levels_us": round(gaps / max(nb, 1) * 1e3, 2),
-           "between_levels_note": "device time from the end of one level's kernels to the "
-                                  "start of the next level's (host decision and launch; at the "
-                                  "bottom-up -> top-down switch also the frontier conversion)"}
+           "between_steps_us": round(gaps / max(nb, 1) * 1e3, 2),
+           "mispredicted_steps_per_bfs": round(miss / len(roots), 2),
+           "between_steps_note": "device time from the end of one step's kernels (a level, or "
+                                 "the frontier conversion at the bottom-up -> top-down switch) "
+                                 "to the start of the next step's, in runs with level timing; "
+                                 "the next step is queued before the previous one reports, on "
+                                 "the direction the host predicts, and a wrong guess costs "
+                                 "one host round trip"}
     log(f"scale24: {out}")
     bfs.destroy()
     graph.free()
@@ -365,6 +370,7 @@ def main():
     gather_ms = np.zeros(runs)
     sweep_ms = np.zeros(runs)
     sweeps = np.zeros(runs, np.int64)
+    mispredicted = np.zeros(runs, np.int64)
     te_by_root, trace_by_root = {}, {}
     all_valid, traces_repeat = True, True
     for k in range(runs):
@@ -375,6 +381,7 @@ def main():
         st, _ = bfs.stats(1)
         gpu_ms[k], gather_ms[k], sweep_ms[k], sweeps[k] = (st.gpu_ms, st.gather_ms,
                                                            st.sweep_ms, st.sweeps)
+        mispredicted[k] = st.mispredicted
         if k < nr:
             trace_by_root[r] = frontier_trace(bfs)
             if args.no_validate:
@@ -502,6 +509,7 @@ def main():
             "gpu_event_hm_gteps": round(gpu_teps_hm / 1e9, 3),
             "median_bfs_ms": round(float(np.median(times)) * 1e3, 4),
             "mean_bfs_gpu_ms": round(mean_gpu_s * 1e3, 4),
+            "mispredicted_steps_per_bfs": round(float(np.mean(mispredicted)), 3),
             "generation_s": round(t_gen, 3), "construction_s": round(t_con, 3),
         },
         "roofline": {

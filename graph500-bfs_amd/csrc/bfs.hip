@@ -35,18 +35,27 @@
 //                 vertices with edges and starts on a second stream at the
 //                 bottom-up -> top-down switch, against a bitmap snapshot;
 //                 k_pred_fixup adds the vertices the last levels reach.
-//   k_level_report, k_queue_report   one block per level: the level's
-//                 totals to the host.
+//   k_level_decide, k_queue_decide   one block closing each step: the
+//                 step's totals, the direction rule, the next step's code.
 //
-// Level hand-off: every block of a level's kernels writes its totals to a
-// per-block slot; a one-block report kernel queued behind them sums the
-// slots, posts the totals and a sequence number to pinned host memory and
-// clears the level counters.  The host
-// spins on the sequence number, decides the next direction exactly as the
-// reference does, and launches the next level -- no copies, memsets or
-// stream synchronisations per level.
+// The level loop is a sequence of steps (a level, or the bitmap -> queue
+// conversion at the bottom-up -> top-down switch).  Every block of a step's
+// kernels writes its totals to a per-block slot; the one-block closing
+// kernel sums the slots, applies the reference's direction rule on the
+// device, writes the code of the step that must run next into the loop
+// state and posts the totals with that code to pinned host memory.  Every
+// kernel of a step first compares its own code with that state and returns
+// at once when they differ.  So the host queues the step it predicts behind
+// the running one before that one reports (no idle GPU between levels when
+// the guess is right); when the posted code says otherwise, the queued
+// guess returns at once and the host queues the right step.  The host also
+// applies the rule to the posted totals (auto_tuning_data, the trace) and
+// checks the device's choice against it.  No copies, memsets or stream
+// synchronisations per level.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <deque>
 #include <cstdlib>
 #include <cstring>
 #include <utility>
@@ -64,6 +73,14 @@ constexpr int kTdItems = 8;
 constexpr int kTdTile = kT * kTdItems;
 constexpr int kTdWin = 512;  // owners staged in LDS per tile (more: read from memory)
 constexpr int kMaxGrid = 2048;
+
+// The top-down step for a frontier of qlen vertices with degree sum qedges:
+// run as a sweep plus conversion past the execution threshold, else with or
+// without the prefix-sum preparation.
+__host__ __device__ inline uint32_t td_step_mode(long long qlen, long long qedges,
+                                                 long long exec_thr) {
+  return qedges > exec_thr ? kModeTDX : (qlen <= kTdWin ? kModeTDS : kModeTD);
+}
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -111,28 +128,88 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t* wsum,
   return x - v + off;
 }
 
-// Level totals and their sequence number to the host's pinned report.
-__device__ inline void post_report(HostReport* hr, const unsigned long long v[4],
+// Step totals to the host's pinned report ring (slot seq % kReportSlots),
+// each word tagged with the low bits of seq.  The host needs nothing else
+// the step wrote, so the words go out as plain system-scope stores: no
+// release fence (which would write the L2 back) on the device's path.
+__device__ inline void post_report(HostReport* ring, const unsigned long long v[4],
                                    unsigned long long seq) {
+  HostReport* hr = ring + (seq % kReportSlots);
+  const unsigned long long tag = seq & ((1ull << kReportTagBits) - 1);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    __hip_atomic_store(&hr->v[i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __threadfence_system();
-  __hip_atomic_store(&hr->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&hr->w[i], (v[i] << kReportTagBits) | tag, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// One block, launched after the level's kernels: posts the level totals --
-// the blocks' partial sums (counts in cpart, degree sums in epart) plus what
-// queue reservations took from ls->cnt -- and clears ls->cnt for the next
-// level.  The blocks of a level kernel finish together, so summing their
-// totals here replaces same-address atomics, which serialize.  hr may be
-// null (totals not needed, counters still cleared).
-__global__ __launch_bounds__(kT) void k_level_report(const uint64_t* __restrict__ cpart,
+// Every kernel of a step starts with this test: the step runs only if the
+// level loop's state names it as the next one.  The closing kernel of the
+// step before wrote that code (kernels of one stream see the writes of the
+// kernels before them), and nothing writes it while the step runs.
+__device__ inline bool step_is_next(const LevelSync* ls, uint32_t code) {
+  return __builtin_amdgcn_readfirstlane(ls->step) == code;
+}
+
+// The direction rule of BfsBase::run_bfs_core (mpi/bfs.hpp:3413-3457) applied
+// on the device after a level with totals (nq_size, nq_edges): top-down ->
+// bottom-up when the frontier grew and its degree sum passes int64(m/alpha);
+// bottom-up -> top-down (for good) once it stops growing and holds fewer than
+// int64(n_c/(beta*ef*2)) vertices.  Picks the next step -- a top-down level
+// with more than m/exec_alpha frontier edges runs as a sweep plus conversion
+// (kModeTDX); a bottom-up level walks the unvisited lists unless a top-down
+// expansion ran last; the switch back to top-down starts with the
+// conversion -- writes its code and returns it.  One thread.
+__device__ inline uint32_t decide_next(const LevelSync& cur, LevelSync* ls, uint32_t code,
+                                       long long nq_size, long long nq_edges) {
+  const uint32_t mode = step_mode(code);
+  const bool td = mode_is_td(mode);
+  uint32_t next;
+  if (nq_size == 0) {
+    next = kModeDone;
+  } else {
+    bool next_td = td, growing = cur.growing != 0;
+    if (growing && nq_size > cur.prev_nq) {
+      if (td && nq_edges > cur.alpha_thr) next_td = false;
+    } else if (!td && nq_size < cur.beta_thr) {
+      next_td = true;
+      growing = false;
+    }
+    ls->growing = growing ? 1u : 0u;
+    ls->prev_nq = nq_size;
+    if (td && next_td) {
+      next = td_step_mode(nq_size, nq_edges, cur.exec_thr);
+      ls->qlen = nq_size;
+      ls->qedges = nq_edges;
+    } else if (td) {
+      // a top-down expansion leaves the unvisited lists behind, a sweep
+      // (also one that ran a top-down level) refreshes them
+      next = mode == kModeTDX ? kModeBUList : kModeBUSweep;
+    } else {
+      next = next_td ? kModeConv : kModeBUList;
+    }
+  }
+  const uint32_t nc = step_code(step_level(code) + 1, next);
+  ls->step = nc;
+  return nc;
+}
+
+// One block, launched after a level's kernels (step `code`): sums the
+// blocks' partial totals (counts in cpart, degree sums in epart) plus what
+// queue reservations took from ls->cnt, clears ls->cnt, decides the next
+// step and posts the totals with its code.  The blocks of a level kernel
+// finish together, so summing their totals here replaces same-address
+// atomics, which serialize.  Thread 0 reads the loop state at the start
+// (the earlier kernels' atomics are complete at this kernel's start), so
+// its loads overlap the partial sums'.
+__global__ __launch_bounds__(kT) void k_level_decide(const uint64_t* __restrict__ cpart,
                                                     int nc,
                                                     const uint64_t* __restrict__ epart,
                                                     int ne, LevelSync* ls, HostReport* hr,
-                                                    unsigned long long seq) {
+                                                    unsigned long long seq, uint32_t code) {
+  if (!step_is_next(ls, code)) return;
   __shared__ uint64_t red[kWavesPerBlock];
+  LevelSync cur;
+  if (threadIdx.x == 0) cur = *ls;
   uint64_t c = 0, d = 0;
   for (int i = threadIdx.x; i < nc; i += kT) c += cpart[i];
   for (int i = threadIdx.x; i < ne; i += kT) d += epart[i];
@@ -143,24 +220,36 @@ __global__ __launch_bounds__(kT) void k_level_report(const uint64_t* __restrict_
     unsigned long long v[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      v[i] = __hip_atomic_load(&ls->cnt[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ls->cnt[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v[i] = cur.cnt[i];
+      ls->cnt[i] = 0;
     }
     v[0] += c;
     v[1] += d;
-    if (hr) post_report(hr, v, seq);
+    v[2] = decide_next(cur, ls, code, (long long)v[0], (long long)v[1]);
+    v[3] = 0;
+    post_report(hr, v, seq);
   }
 }
+
+// Per-root setup of the level loop: first step, frontier, thresholds (the
+// first frontier's degree sum is the root's row length, set on the device).
+struct LoopInit {
+  uint32_t first;  // code of the first step
+  long long qlen, qedges, alpha_thr, beta_thr, exec_thr;
+};
 
 // Bits of vertices >= n_c are pre-set so they are never searched
 // (the reference pre-visits vertices without edges, mpi/bfs.hpp:330-334).
 // Per-root init in one launch: the bitmap (plus the root's bit) and, from
-// thread 0, the root's pred entry, the one-vertex frontier and cleared level
-// counters (a run that failed inside a level leaves its reservations there).
+// thread 0, the root's pred entry, the one-vertex frontier and the level
+// loop's state (cleared counters -- a run that failed inside a level leaves
+// its reservations there -- and the first step).
 __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
                                                 int64_t words, int64_t n_c,
                                                 int64_t root_new, int64_t* pred_new,
-                                                int64_t entry, uint32_t* q, LevelSync* ls) {
+                                                int64_t entry, uint32_t* q, LevelSync* ls,
+                                                LoopInit li,
+                                                const int64_t* __restrict__ row_off) {
   for (int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x; w < words;
        w += (int64_t)gridDim.x * kT) {
     const int64_t lo = w * 64;
@@ -173,9 +262,18 @@ __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
   if (blockIdx.x == 0 && threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) ls->cnt[i] = 0;
+    ls->step = li.first;
+    ls->growing = 1u;
+    ls->qlen = li.qlen;
+    ls->qedges = li.qedges;
+    ls->prev_nq = 0;
+    ls->alpha_thr = li.alpha_thr;
+    ls->beta_thr = li.beta_thr;
+    ls->exec_thr = li.exec_thr;
     if (root_new >= 0 && root_new < n_c) {
       pred_new[root_new] = entry;
       q[0] = (uint32_t)root_new;
+      ls->qedges = row_off[root_new + 1] - row_off[root_new];
     }
   }
 }
@@ -217,35 +315,46 @@ __device__ inline int64_t owner_search(const uint64_t* a, int64_t lo,
 constexpr int kPrepItems = 8;
 constexpr int kPrepTile = kT * kPrepItems;
 
+// The prep kernels run on a fixed grid (the frontier's size is on the
+// device when they are queued): block b takes blocks b, b + grid, ... of
+// the frontier.  Both return at once for a frontier of up to kTdWin vertices.
+constexpr int kPrepGrid = 256;
+
 // Block-local prefix sums into dsum, block totals into bsum; the last block
 // to finish (atomic ticket, which it resets) scans the totals into boff.
 __global__ __launch_bounds__(kT) void k_td_prep_sums(
-    const uint32_t* __restrict__ q, int64_t qlen, const int64_t* __restrict__ row_off,
-    uint64_t* __restrict__ dsum, uint64_t* __restrict__ bsum, uint64_t* __restrict__ boff,
-    unsigned int* ticket) {
+    const uint32_t* __restrict__ q, const LevelSync* ls, uint32_t code,
+    const int64_t* __restrict__ row_off, uint64_t* __restrict__ dsum,
+    uint64_t* __restrict__ bsum, uint64_t* __restrict__ boff, unsigned int* ticket) {
+  if (!step_is_next(ls, code)) return;
+  const int64_t qlen = ls->qlen;
+  if (qlen <= kTdWin) return;
   __shared__ uint64_t wsum[kWavesPerBlock];
   __shared__ bool last;
   const int t = threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * kPrepTile + (int64_t)t * kPrepItems;
-  uint32_t u[kPrepItems];
+  const int64_t nb = ceil_div(qlen, (int64_t)kPrepTile);
+  for (int64_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    const int64_t i0 = blk * kPrepTile + (int64_t)t * kPrepItems;
+    uint32_t u[kPrepItems];
 #pragma unroll
-  for (int j = 0; j < kPrepItems; ++j) u[j] = i0 + j < qlen ? q[i0 + j] : 0u;
-  uint64_t d[kPrepItems];
+    for (int j = 0; j < kPrepItems; ++j) u[j] = i0 + j < qlen ? q[i0 + j] : 0u;
+    uint64_t d[kPrepItems];
 #pragma unroll
-  for (int j = 0; j < kPrepItems; ++j)
-    d[j] = i0 + j < qlen ? (uint64_t)(row_off[u[j] + 1] - row_off[u[j]]) : 0u;
-  uint64_t sum = 0;
+    for (int j = 0; j < kPrepItems; ++j)
+      d[j] = i0 + j < qlen ? (uint64_t)(row_off[u[j] + 1] - row_off[u[j]]) : 0u;
+    uint64_t sum = 0;
 #pragma unroll
-  for (int j = 0; j < kPrepItems; ++j) sum += d[j];
-  uint64_t tot;
-  uint64_t a = block_excl_scan(sum, wsum, &tot);
+    for (int j = 0; j < kPrepItems; ++j) sum += d[j];
+    uint64_t tot;
+    uint64_t a = block_excl_scan(sum, wsum, &tot);
 #pragma unroll
-  for (int j = 0; j < kPrepItems; ++j) {
-    if (i0 + j < qlen) dsum[i0 + j] = a;
-    a += d[j];
+    for (int j = 0; j < kPrepItems; ++j) {
+      if (i0 + j < qlen) dsum[i0 + j] = a;
+      a += d[j];
+    }
+    if (t == 0) bsum[blk] = tot;
   }
   if (t == 0) {
-    bsum[blockIdx.x] = tot;
     __threadfence();
     last = atomicAdd(ticket, 1u) == gridDim.x - 1;
   }
@@ -253,7 +362,6 @@ __global__ __launch_bounds__(kT) void k_td_prep_sums(
   if (!last) return;
   __threadfence();
   uint64_t carry = 0;
-  const int64_t nb = gridDim.x;
   for (int64_t b0 = 0; b0 < nb; b0 += kT) {
     const uint64_t x = b0 + t < nb ? __atomic_load_n(&bsum[b0 + t], __ATOMIC_RELAXED) : 0;
     uint64_t ctot;
@@ -269,46 +377,60 @@ __global__ __launch_bounds__(kT) void k_td_prep_sums(
 
 // Adds the block offsets and writes the tile owners.
 __global__ __launch_bounds__(kT) void k_td_prep_tiles(
-    int64_t qlen, uint64_t* __restrict__ dsum, const uint64_t* __restrict__ bsum,
-    const uint64_t* __restrict__ boff, uint32_t* __restrict__ tile_owner) {
+    const LevelSync* ls, uint32_t code, uint64_t* __restrict__ dsum,
+    const uint64_t* __restrict__ bsum, const uint64_t* __restrict__ boff,
+    uint32_t* __restrict__ tile_owner) {
+  if (!step_is_next(ls, code)) return;
+  const int64_t qlen = ls->qlen;
+  if (qlen <= kTdWin) return;
   const int t = threadIdx.x;
-  const int64_t blk = blockIdx.x;
-  const int64_t i0 = blk * kPrepTile + (int64_t)t * kPrepItems;
-  const int64_t bend = min((blk + 1) * (int64_t)kPrepTile, qlen);
-  // local prefix sums of this thread's entries and of the entry after them
-  // (the next thread's first, or the block total), read before any write
-  uint64_t loc[kPrepItems + 1];
+  const int64_t nb = ceil_div(qlen, (int64_t)kPrepTile);
+  for (int64_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    const int64_t i0 = blk * kPrepTile + (int64_t)t * kPrepItems;
+    const int64_t bend = min((blk + 1) * (int64_t)kPrepTile, qlen);
+    // local prefix sums of this thread's entries and of the entry after them
+    // (the next thread's first, or the block total), read before any write
+    uint64_t loc[kPrepItems + 1];
 #pragma unroll
-  for (int j = 0; j <= kPrepItems; ++j) {
-    const int64_t i = i0 + j;
-    loc[j] = i < bend ? dsum[i] : bsum[blk];
-  }
-  const uint64_t off = boff[blk];
-  __syncthreads();
+    for (int j = 0; j <= kPrepItems; ++j) {
+      const int64_t i = i0 + j;
+      loc[j] = i < bend ? dsum[i] : bsum[blk];
+    }
+    const uint64_t off = boff[blk];
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kPrepItems; ++j) {
-    const int64_t i = i0 + j;
-    if (i >= bend) break;
-    const uint64_t a = off + loc[j], b = off + loc[j + 1];
-    dsum[i] = a;
-    if (b <= a) continue;
-    const uint64_t t0 = (a + kTdTile - 1) / kTdTile, t1 = (b - 1) / kTdTile;
-    for (uint64_t tt = t0; tt <= t1; ++tt) tile_owner[tt] = (uint32_t)i;
-    if (i == qlen - 1) tile_owner[t1 + 1] = (uint32_t)i;
+    for (int j = 0; j < kPrepItems; ++j) {
+      const int64_t i = i0 + j;
+      if (i >= bend) break;
+      const uint64_t a = off + loc[j], b = off + loc[j + 1];
+      dsum[i] = a;
+      if (b <= a) continue;
+      const uint64_t t0 = (a + kTdTile - 1) / kTdTile, t1 = (b - 1) / kTdTile;
+      for (uint64_t tt = t0; tt <= t1; ++tt) tile_owner[tt] = (uint32_t)i;
+      if (i == qlen - 1) tile_owner[t1 + 1] = (uint32_t)i;
+    }
   }
 }
 
 // Frontiers of at most kTdWin vertices need no preparation kernels: every
 // block stages the whole frontier (prefix sum of degrees, row base, original
 // ID) in LDS itself.
+// The grid is fixed (kMaxGrid); blocks past the frontier's degree sum (an
+// upper bound of its row entries) write a zero total and return.
 __global__ __launch_bounds__(kT) void k_td_expand(
-    const uint32_t* __restrict__ q, int64_t qlen,
+    const uint32_t* __restrict__ q, uint32_t code,
     const uint64_t* __restrict__ dsum, const uint32_t* __restrict__ tile_owner,
     const int64_t* __restrict__ row_off,
     const uint32_t* __restrict__ col, const uint32_t* __restrict__ n2o,
     const uint32_t* __restrict__ deg_new, unsigned long long* vis,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ nq, LevelSync* ls,
     uint64_t* __restrict__ epart, uint64_t level_bits) {
+  if (!step_is_next(ls, code)) return;
+  if ((uint64_t)blockIdx.x * kTdTile >= (uint64_t)ls->qedges) {
+    if (threadIdx.x == 0) epart[blockIdx.x] = 0;
+    return;
+  }
+  const int64_t qlen = ls->qlen;
   // the tile's owners, staged once per tile: prefix sum, col offset of edge
   // 0 of the owner's range (row_off[u] - dsum[i]) and original ID.  On heavy
   // levels a tile has one or two owners (hubs) shared by every block at
@@ -597,7 +719,9 @@ __global__ __launch_bounds__(kBuT) void k_bu_sweep(
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
     uint32_t* __restrict__ unv_cnt, uint32_t* __restrict__ longv, LevelSync* ls,
-    uint64_t* __restrict__ cpart, uint64_t level_bits, uint64_t* __restrict__ wacc) {
+    uint32_t code, uint64_t* __restrict__ cpart, uint64_t level_bits,
+    uint64_t* __restrict__ wacc) {
+  if (!step_is_next(ls, code)) return;
   __shared__ unsigned long long fw[kBuWaves][kWave];
   __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
@@ -786,8 +910,9 @@ __global__ __launch_bounds__(kBuT) void k_bu_long(
     const int64_t* __restrict__ row_off, const uint32_t* __restrict__ col,
     const uint32_t* __restrict__ n2o, int64_t* __restrict__ pred_new,
     uint32_t* __restrict__ unv, uint32_t* unv_cnt,
-    const uint32_t* __restrict__ longv, const LevelSync* ls,
+    const uint32_t* __restrict__ longv, const LevelSync* ls, uint32_t code,
     uint64_t* __restrict__ cpart, uint64_t level_bits, uint64_t* __restrict__ wacc) {
+  if (!step_is_next(ls, code)) return;
   __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
   const VisProbe visited = stage_hubs(hub, vis_old, words);
@@ -859,8 +984,9 @@ __global__ __launch_bounds__(kBuT) __attribute__((amdgpu_waves_per_eu(4))) void 
     const uint32_t* __restrict__ col, const uint2* __restrict__ first,
     const uint64_t* __restrict__ multi, const uint32_t* __restrict__ n2o,
     int64_t* __restrict__ pred_new, uint32_t* __restrict__ unv,
-    uint32_t* __restrict__ unv_cnt, uint64_t* __restrict__ cpart, uint64_t level_bits,
-    uint64_t* __restrict__ wacc) {
+    uint32_t* __restrict__ unv_cnt, const LevelSync* ls, uint32_t code,
+    uint64_t* __restrict__ cpart, uint64_t level_bits, uint64_t* __restrict__ wacc) {
+  if (!step_is_next(ls, code)) return;
   __shared__ unsigned long long fw[kBuWaves][kWave];
   __shared__ uint64_t red[kBuWaves];
   __shared__ uint64_t hub[kHubWords];
@@ -971,7 +1097,9 @@ __device__ inline void queue_wave_span(int64_t words, int64_t* a, int64_t* e) {
 __global__ __launch_bounds__(kT) void k_queue_count(const uint64_t* __restrict__ cur,
                                                    const uint64_t* __restrict__ prev,
                                                    int64_t words, uint32_t* __restrict__ wcnt,
-                                                   uint64_t* __restrict__ snap) {
+                                                   uint64_t* __restrict__ snap,
+                                                   const LevelSync* ls, uint32_t code) {
+  if (!step_is_next(ls, code)) return;
   int64_t a, e;
   queue_wave_span(words, &a, &e);
   uint64_t c = 0;
@@ -990,7 +1118,9 @@ __global__ __launch_bounds__(kT) void k_queue_write(const uint64_t* __restrict__
                                                    const uint32_t* __restrict__ deg_new,
                                                    const uint32_t* __restrict__ wcnt,
                                                    uint32_t* __restrict__ out,
-                                                   uint64_t* __restrict__ bedges) {
+                                                   uint64_t* __restrict__ bedges,
+                                                   const LevelSync* ls, uint32_t code) {
+  if (!step_is_next(ls, code)) return;
   __shared__ uint64_t red[kWavesPerBlock];
   const int t = threadIdx.x, lane = lane_id(), wave = t / kWave;
   const int64_t first = (int64_t)blockIdx.x * kWavesPerBlock;
@@ -1036,14 +1166,23 @@ __global__ __launch_bounds__(kT) void k_queue_write(const uint64_t* __restrict__
   if (t == 0) bedges[blockIdx.x] = s;
 }
 
-// One block: the conversion's totals (count, degree sum) to the host.  All
-// blocks of k_queue_write finish together, so their totals are summed here
-// rather than by atomics on one address, which serialize.
-__global__ __launch_bounds__(kT) void k_queue_report(const uint32_t* __restrict__ wcnt,
+// One block, closing a step whose frontier came from the bitmap: sums the
+// conversion's totals (count, degree sum) -- all blocks of k_queue_write
+// finish together, so here rather than by atomics on one address, which
+// serialize -- and clears ls->cnt (a top-down level run as a sweep leaves
+// the long-row count there).  After a top-down level run as a sweep the
+// totals are the level's and the direction rule decides the next step; after
+// the conversion at the switch to top-down they are the first top-down
+// frontier's, which runs next (as a sweep if its degree sum passes the
+// execution threshold).
+__global__ __launch_bounds__(kT) void k_queue_decide(const uint32_t* __restrict__ wcnt,
                                                     const uint64_t* __restrict__ bedges,
                                                     int blocks, LevelSync* ls, HostReport* hr,
-                                                    unsigned long long seq) {
+                                                    unsigned long long seq, uint32_t code) {
+  if (!step_is_next(ls, code)) return;
   __shared__ uint64_t red[kWavesPerBlock];
+  LevelSync cur;
+  if (threadIdx.x == 0) cur = *ls;
   uint64_t c = 0, d = 0;
   for (int i = threadIdx.x; i < blocks * kWavesPerBlock; i += kT) c += wcnt[i];
   for (int i = threadIdx.x; i < blocks; i += kT) d += bedges[i];
@@ -1051,11 +1190,18 @@ __global__ __launch_bounds__(kT) void k_queue_report(const uint32_t* __restrict_
   __syncthreads();  // thread 0 has read red
   d = block_sum(d, red);
   if (threadIdx.x == 0) {
-    // a top-down level run as a sweep leaves the long-row count behind
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      __hip_atomic_store(&ls->cnt[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long v[4] = {c, d, 0ull, 0ull};
+    for (int i = 0; i < 4; ++i) ls->cnt[i] = 0;
+    uint32_t next;
+    if (step_mode(code) == kModeConv) {
+      ls->qlen = (long long)c;
+      ls->qedges = (long long)d;
+      next = step_code(step_level(code), td_step_mode((long long)c, (long long)d, cur.exec_thr));
+      ls->step = next;
+    } else {
+      next = decide_next(cur, ls, code, (long long)c, (long long)d);
+    }
+    const unsigned long long v[4] = {c, d, next, 0ull};
     post_report(hr, v, seq);
   }
 }
@@ -1224,9 +1370,9 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   b->wacc.alloc((size_t)b->bu_grid[0] + b->bu_grid[2] + b->bu_grid[1]);
   b->lsync.alloc(1);
   G500_CHECK_HIP(hipMemset(b->lsync.get(), 0, sizeof(LevelSync)));
-  G500_CHECK_HIP(hipHostMalloc((void**)&b->h_report, sizeof(HostReport),
+  G500_CHECK_HIP(hipHostMalloc((void**)&b->h_report, kReportSlots * sizeof(HostReport),
                                hipHostMallocMapped | hipHostMallocCoherent));
-  std::memset((void*)b->h_report, 0, sizeof(HostReport));
+  std::memset((void*)b->h_report, 0, kReportSlots * sizeof(HostReport));
   G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
@@ -1267,6 +1413,7 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   }
   ensure_host_orig2new(g);
   if (const char* ea = getenv("G500_EXEC_ALPHA")) b->exec_alpha = atof(ea);
+  if (const char* sp = getenv("G500_SPECULATE")) b->spec_depth = std::max(0, atoi(sp));
   if (g->corebfs && !g->core_reset_done) {
     // corebfs_index::reset_unreachable: one BFS over the core from its
     // highest-degree vertex decides which forest parents survive
@@ -1282,30 +1429,38 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
 
 namespace {
 
-// Spins until the GPU posted report `seq`; returns its four totals.
+// Spins until the GPU posted report `seq` (slot seq % kReportSlots); returns
+// its four values.
 void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4]) {
-  HostReport* hr = b->h_report;
+  HostReport* hr = b->h_report + (seq % kReportSlots);
+  const unsigned long long tag = seq & ((1ull << kReportTagBits) - 1);
+  const unsigned long long tmask = (1ull << kReportTagBits) - 1;
+  auto whole = [&]() {
+    for (int i = 0; i < 4; ++i) {
+      const unsigned long long w = __atomic_load_n(&hr->w[i], __ATOMIC_ACQUIRE);
+      if ((w & tmask) != tag) return false;
+      out[i] = w >> kReportTagBits;
+    }
+    return true;
+  };
   uint64_t spins = 0;
   const auto t0 = std::chrono::steady_clock::now();
-  while (__atomic_load_n(&hr->seq, __ATOMIC_ACQUIRE) != seq) {
+  while (!whole()) {
     if ((++spins & 1023) == 0) {
       const hipError_t q = hipStreamQuery(b->stream);
       if (q == hipSuccess) {
-        if (__atomic_load_n(&hr->seq, __ATOMIC_ACQUIRE) == seq) break;
-        fail(G500_ERR_STATE, "level report %llu missing after the stream drained", seq);
+        if (whole()) break;
+        fail(G500_ERR_STATE, "step report %llu missing after the stream drained", seq);
       }
       if (q != hipErrorNotReady)
         fail(G500_ERR_HIP, "stream error while waiting for a level: %s",
              hipGetErrorString(q));
       if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300))
-        fail(G500_ERR_STATE, "timed out waiting for level report %llu", seq);
+        fail(G500_ERR_STATE, "timed out waiting for step report %llu", seq);
     }
   }
-  for (int i = 0; i < 4; ++i) out[i] = __atomic_load_n(&hr->v[i], __ATOMIC_ACQUIRE);
 }
 
-// One bottom-up level: vis -> vis_alt, then the two swap.  `lists` walks
-// the unvisited lists the previous bottom-up level left, else the bitmap.
 // pred by original ID from the entries of the vertices set in `vis` (a
 // smaller grid beside the last top-down levels measured slower: the random
 // reads need every wave the chip holds).
@@ -1328,72 +1483,274 @@ void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t s
   G500_CHECK_HIP(hipEventRecord(b->ev_g1, st));
 }
 
-// The sweep lists its long rows in nq, which no bottom-up level uses (a
-// top-down level run as a sweep refills it from the bitmap afterwards).
-void launch_bu(g500_bfs* b, bool lists, unsigned long long seq, uint64_t level_bits,
-               bool report) {
+// The buffers a step starts from: the visited bitmap, the other bitmap (the
+// one before the last level), the frontier queue and the next queue.
+struct Bufs {
+  uint64_t* vis;
+  uint64_t* vis_alt;
+  uint32_t* q;
+  uint32_t* nq;
+};
+
+// A queued step: its code, report number, the buffers after it (if it
+// runs) and its event pairs.
+struct Step {
+  uint32_t code = 0;
+  unsigned long long seq = 0;
+  Bufs after{};
+  int ev = -1;        // level_events pair (timing runs)
+  int sweep_ev = -1;  // sweep_events pair (steps with a sweep)
+};
+
+hipEvent_t pool_event(std::vector<hipEvent_t>& pool, size_t i) {
+  while (pool.size() <= i) {
+    hipEvent_t e;
+    G500_CHECK_HIP(hipEventCreate(&e));
+    pool.push_back(e);
+  }
+  return pool[i];
+}
+
+// Queues step `code` behind whatever the engine stream holds; `in` are the
+// buffers it starts from.  Every kernel of the step returns at once unless
+// the level loop's state names this step next.  A bottom-up level writes
+// the other bitmap (the two swap roles) and lists its long rows in nq, which
+// no bottom-up level uses (a top-down level run as a sweep refills it from
+// the bitmap afterwards); a top-down level swaps the queues; the conversion
+// at the switch to top-down writes the queue and the gather's bitmap
+// snapshot (ev_snap is recorded as soon as that is written).
+Step launch_step(g500_bfs* b, uint32_t code, const Bufs& in, size_t* nlev_ev,
+                 size_t* nsweep_ev) {
   g500_graph* g = b->g;
   hipStream_t s = b->stream;
-  int parts = 0;
+  LevelSync* ls = b->lsync.get();
+  Step st;
+  st.code = code;
+  st.seq = ++b->seq;
+  st.after = in;
+  const uint32_t mode = step_mode(code);
+  const uint64_t level_bits = (uint64_t)step_level(code) << 48;
+  if (b->timing) {
+    st.ev = (int)*nlev_ev;
+    *nlev_ev += 2;
+    G500_CHECK_HIP(hipEventRecord(pool_event(b->level_events, st.ev), s));
+  }
   // counting runs: per-block scan bytes, sweep / long-row / list slots
   uint64_t* wsweep = b->counting ? b->wacc.get() : nullptr;
   uint64_t* wlong = b->counting ? b->wacc.get() + b->bu_grid[0] : nullptr;
   uint64_t* wlist = b->counting ? b->wacc.get() + b->bu_grid[0] + b->bu_grid[2] : nullptr;
-  if (lists) {
-    hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kBuT), 0, s, b->vis.get(),
-                       b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
-                       g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
-                       b->unv.get(), b->unv_cnt.get(), b->lpart.get(), level_bits, wlist);
+  auto sweep = [&]() {
+    st.sweep_ev = (int)*nsweep_ev;
+    *nsweep_ev += 2;
+    G500_CHECK_HIP(hipEventRecord(pool_event(b->sweep_events, st.sweep_ev), s));
+    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kBuT), 0, s, in.vis, in.vis_alt,
+                       b->words, g->row_off.get(), g->col.get(), g->first.get(),
+                       g->multi.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
+                       b->unv_cnt.get(), in.nq, ls, code, b->lpart.get(), level_bits, wsweep);
     G500_CHECK_LAUNCH();
-    parts = (int)b->bu_grid[1];
-  } else {
-    const size_t k = 2 * (size_t)b->nsweeps++;
-    while (b->sweep_events.size() < k + 2) {
-      hipEvent_t e;
-      G500_CHECK_HIP(hipEventCreate(&e));
-      b->sweep_events.push_back(e);
-    }
-    G500_CHECK_HIP(hipEventRecord(b->sweep_events[k], s));
-    hipLaunchKernelGGL(k_bu_sweep, dim3(b->bu_grid[0]), dim3(kBuT), 0, s, b->vis.get(),
-                       b->vis_alt.get(), b->words, g->row_off.get(), g->col.get(),
-                       g->first.get(), g->multi.get(), g->new2orig.get(), b->pred_new.get(),
-                       b->unv.get(), b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
-                       b->lpart.get(), level_bits, wsweep);
-    G500_CHECK_LAUNCH();
-    G500_CHECK_HIP(hipEventRecord(b->sweep_events[k + 1], s));
-    hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kBuT), 0, s, b->vis.get(),
-                       (unsigned long long*)b->vis_alt.get(), b->words, g->row_off.get(),
+    G500_CHECK_HIP(hipEventRecord(pool_event(b->sweep_events, st.sweep_ev + 1), s));
+    hipLaunchKernelGGL(k_bu_long, dim3(b->bu_grid[2]), dim3(kBuT), 0, s, in.vis,
+                       (unsigned long long*)in.vis_alt, b->words, g->row_off.get(),
                        g->col.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
-                       b->unv_cnt.get(), b->nq.get(), b->lsync.get(),
-                       b->lpart.get() + b->bu_grid[0], level_bits, wlong);
+                       b->unv_cnt.get(), in.nq, ls, code, b->lpart.get() + b->bu_grid[0],
+                       level_bits, wlong);
     G500_CHECK_LAUNCH();
-    parts = (int)(b->bu_grid[0] + b->bu_grid[2]);
-  }
-  if (report) {
-    hipLaunchKernelGGL(k_level_report, dim3(1), dim3(kT), 0, s, b->lpart.get(), parts,
-                       (const uint64_t*)nullptr, 0, b->lsync.get(), b->d_report, seq);
+    st.after.vis = in.vis_alt;
+    st.after.vis_alt = in.vis;
+  };
+  // bitmap -> queue: the bits set in cur but not in prev
+  auto to_queue = [&](const uint64_t* cur, const uint64_t* prev, uint32_t* out,
+                      uint64_t* snap) {
+    const dim3 grid((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(b->words, (int64_t)kT)));
+    hipLaunchKernelGGL(k_queue_count, grid, dim3(kT), 0, s, cur, prev, b->words, b->qcnt.get(),
+                       snap, ls, code);
     G500_CHECK_LAUNCH();
+    if (snap) G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
+    hipLaunchKernelGGL(k_queue_write, grid, dim3(kT), 0, s, cur, prev, b->words,
+                       g->deg_new.get(), b->qcnt.get(), out, b->qedges.get(), ls, code);
+    G500_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_queue_decide, dim3(1), dim3(kT), 0, s, b->qcnt.get(), b->qedges.get(),
+                       (int)grid.x, ls, b->d_report, st.seq, code);
+    G500_CHECK_LAUNCH();
+  };
+  switch (mode) {
+    case kModeTD:
+    case kModeTDS:
+      if (mode == kModeTD) {
+        hipLaunchKernelGGL(k_td_prep_sums, dim3(kPrepGrid), dim3(kT), 0, s, in.q, ls, code,
+                           g->row_off.get(), b->dsum.get(), b->bsum.get(), b->boff.get(),
+                           b->prep_ticket.get());
+        G500_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_td_prep_tiles, dim3(kPrepGrid), dim3(kT), 0, s, ls, code,
+                           b->dsum.get(), b->bsum.get(), b->boff.get(), b->tile_owner.get());
+        G500_CHECK_LAUNCH();
+      }
+      hipLaunchKernelGGL(k_td_expand, dim3(kMaxGrid), dim3(kT), 0, s, in.q, code,
+                         b->dsum.get(), b->tile_owner.get(), g->row_off.get(), g->col.get(),
+                         g->new2orig.get(), g->deg_new.get(), (unsigned long long*)in.vis,
+                         b->pred_new.get(), in.nq, ls, b->lpart.get(), level_bits);
+      G500_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_level_decide, dim3(1), dim3(kT), 0, s, (const uint64_t*)nullptr, 0,
+                         b->lpart.get(), kMaxGrid, ls, b->d_report, st.seq, code);
+      G500_CHECK_LAUNCH();
+      std::swap(st.after.q, st.after.nq);
+      break;
+    case kModeTDX:
+      sweep();
+      to_queue(in.vis_alt, in.vis, in.nq, nullptr);
+      std::swap(st.after.q, st.after.nq);
+      break;
+    case kModeBUSweep:
+      sweep();
+      hipLaunchKernelGGL(k_level_decide, dim3(1), dim3(kT), 0, s, b->lpart.get(),
+                         (int)(b->bu_grid[0] + b->bu_grid[2]), (const uint64_t*)nullptr, 0, ls,
+                         b->d_report, st.seq, code);
+      G500_CHECK_LAUNCH();
+      break;
+    case kModeBUList:
+      hipLaunchKernelGGL(k_bu_list, dim3(b->bu_grid[1]), dim3(kBuT), 0, s, in.vis, in.vis_alt,
+                         b->words, g->row_off.get(), g->col.get(), g->first.get(),
+                         g->multi.get(), g->new2orig.get(), b->pred_new.get(), b->unv.get(),
+                         b->unv_cnt.get(), ls, code, b->lpart.get(), level_bits, wlist);
+      G500_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_level_decide, dim3(1), dim3(kT), 0, s, b->lpart.get(),
+                         (int)b->bu_grid[1], (const uint64_t*)nullptr, 0, ls, b->d_report,
+                         st.seq, code);
+      G500_CHECK_LAUNCH();
+      st.after.vis = in.vis_alt;
+      st.after.vis_alt = in.vis;
+      break;
+    case kModeConv:
+      to_queue(in.vis, in.vis_alt, in.q, b->vis_snap.get());
+      break;
+    default:
+      fail(G500_ERR_STATE, "no step of mode %u", mode);
   }
-  std::swap(b->vis.p, b->vis_alt.p);
+  if (b->timing) G500_CHECK_HIP(hipEventRecord(pool_event(b->level_events, st.ev + 1), s));
+  return st;
 }
 
-// Frontier queue from the bits of the level just finished (vis minus
-// vis_alt, the bitmap before it); the level report carries its size and
-// degree sum.
-// With snap, ev_snap is recorded as soon as the snapshot is written.
-void launch_to_queue(g500_bfs* b, uint32_t* out, unsigned long long seq, uint64_t* snap) {
-  hipStream_t s = b->stream;
-  const dim3 grid((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(b->words, (int64_t)kT)));
-  hipLaunchKernelGGL(k_queue_count, grid, dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(),
-                     b->words, b->qcnt.get(), snap);
-  G500_CHECK_LAUNCH();
-  if (snap) G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
-  hipLaunchKernelGGL(k_queue_write, grid, dim3(kT), 0, s, b->vis.get(), b->vis_alt.get(),
-                     b->words, b->g->deg_new.get(), b->qcnt.get(), out, b->qedges.get());
-  G500_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_queue_report, dim3(1), dim3(kT), 0, s, b->qcnt.get(), b->qedges.get(),
-                     (int)grid.x, b->lsync.get(), b->d_report, seq);
-  G500_CHECK_LAUNCH();
+// Host mirror of the level loop (mpi/bfs.hpp:3264-3279, decision at
+// 3413-3457): the same rule as decide_next on the device, applied to each
+// step's reported totals; it fills auto_tuning_data and the trace and
+// checks the device's choice.
+struct HostLoop {
+  bool growing = true;
+  int64_t prev_nq = 0;
+  int64_t alpha_thr = 0, beta_thr = 0, exec_thr = 0;
+  int64_t n_c = 0, m = 0;
+
+  // next step after level step `code` reported (nq_size, nq_edges)
+  uint32_t after_level(uint32_t code, int64_t nq_size, int64_t nq_edges, int64_t* at) {
+    const uint32_t mode = step_mode(code);
+    const int level = step_level(code);
+    const bool td = mode_is_td(mode);
+    if (nq_size == 0) return step_code(level + 1, kModeDone);
+    bool next_td = td;
+    if (growing && nq_size > prev_nq) {
+      if (td) {
+        if (nq_edges > alpha_thr) {
+          next_td = false;
+          at[G500_AT_GLOBAL_NQ_EDGES] = nq_edges;
+          at[G500_AT_NUM_GLOBAL_EDGES] = m;
+          at[G500_AT_T2B_LEVEL] = level;
+        } else {
+          at[G500_AT_PRE_GLOBAL_NQ_EDGES] = nq_edges;
+        }
+      }
+    } else if (!td) {
+      if (nq_size < beta_thr) {
+        next_td = true;
+        growing = false;
+        at[G500_AT_GLOBAL_NQ_SIZE] = nq_size;
+        at[G500_AT_NUM_GLOBAL_VERTS] = n_c;
+        at[G500_AT_B2T_LEVEL] = level;
+      } else {
+        at[G500_AT_PRE_GLOBAL_NQ_SIZE] = nq_size;
+      }
+    }
+    prev_nq = nq_size;
+    uint32_t next;
+    if (td && next_td) next = td_step_mode(nq_size, nq_edges, exec_thr);
+    else if (td) next = mode == kModeTDX ? kModeBUList : kModeBUSweep;
+    else next = next_td ? kModeConv : kModeBUList;
+    return step_code(level + 1, next);
+  }
+};
+
+// Frontier totals (size, degree sum) that a level of mode `mode` is expected
+// to find, extrapolated from the last two levels of `tr` (reported or
+// themselves estimated): the growth from the level before the last one to
+// the last one, applied once more.
+void estimate_level(uint32_t mode, const std::vector<g500_level_trace>& tr, int64_t* nq,
+                    int64_t* fe) {
+  const size_t k = tr.size();
+  auto grow = [&](int64_t a, int64_t b, double dflt) {
+    return (a > 0 && b > 0) ? (double)a / (double)b : dflt;
+  };
+  const double cap = (double)(INT64_C(1) << 50);
+  const int64_t nq0 = k >= 1 ? tr[k - 1].nq_size : 1;
+  const double gq = k >= 2 ? grow(tr[k - 1].nq_size, tr[k - 2].nq_size, 1.0) : 16.0;
+  *nq = std::max<int64_t>(1, (int64_t)std::min((double)nq0 * gq, cap));
+  *fe = 0;
+  if (mode_is_td(mode)) {
+    const int64_t fe0 = k >= 1 ? tr[k - 1].nq_edges : 0;
+    const double ge = k >= 2 ? grow(tr[k - 1].nq_edges, tr[k - 2].nq_edges, 32.0) : 32.0;
+    *fe = (int64_t)std::min((double)fe0 * std::max(ge, 1.0), cap);
+  }
+}
+
+// The step the host queues behind `last` before `last` has reported: the
+// direction rule (a copy of the host mirror) applied to the totals `last` is
+// expected to find.  `tr` holds the reported levels plus the expected ones
+// of the steps queued before `last`.  A wrong guess costs the guessed step's
+// empty launches and one host round trip, never a result.
+uint32_t predict_next(const Step& last, HostLoop hl, const std::vector<g500_level_trace>& tr) {
+  const uint32_t mode = step_mode(last.code);
+  // the conversion's frontier is the last level's
+  if (mode == kModeConv)
+    return step_code(step_level(last.code),
+                     td_step_mode(tr.empty() ? 0 : tr.back().nq_size, 0, hl.exec_thr));
+  int64_t nq, fe, at[G500_AT_NUM];
+  estimate_level(mode, tr, &nq, &fe);
+  return hl.after_level(last.code, nq, fe, at);
+}
+
+// Queues guessed steps behind the queued ones until `depth` are queued:
+// the steps queued after the first one (whose predecessor has reported)
+// contribute their expected totals to the guess of the next one.
+void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t depth,
+               size_t* nlev_ev, size_t* nsweep_ev) {
+  if (pend.empty() || pend.size() >= depth) return;
+  HostLoop h = hl;
+  std::vector<g500_level_trace> tr = b->trace;
+  for (size_t i = 0; i + 1 < pend.size(); ++i) {
+    const uint32_t mode = step_mode(pend[i].code);
+    if (mode == kModeConv) continue;
+    g500_level_trace t{};
+    int64_t at[G500_AT_NUM];
+    estimate_level(mode, tr, &t.nq_size, &t.nq_edges);
+    t.level = step_level(pend[i].code);
+    t.top_down = mode_is_td(mode) ? 1 : 0;
+    (void)h.after_level(pend[i].code, t.nq_size, t.nq_edges, at);
+    tr.push_back(t);
+  }
+  while (pend.size() < depth) {
+    const Step& last = pend.back();
+    const uint32_t next = predict_next(last, h, tr);
+    if (step_mode(next) == kModeDone) return;
+    const uint32_t lm = step_mode(last.code);
+    if (lm != kModeConv) {
+      g500_level_trace t{};
+      int64_t at[G500_AT_NUM];
+      estimate_level(lm, tr, &t.nq_size, &t.nq_edges);
+      t.level = step_level(last.code);
+      t.top_down = mode_is_td(lm) ? 1 : 0;
+      (void)h.after_level(last.code, t.nq_size, t.nq_edges, at);
+      tr.push_back(t);
+    }
+    pend.push_back(launch_step(b, next, last.after, nlev_ev, nsweep_ev));
+  }
 }
 
 }  // namespace
@@ -1441,7 +1798,6 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->trace.clear();
   b->stats = g500_run_stats{};
   b->have_result = false;
-  b->nsweeps = 0;
   // a gather left running by a run that failed must not overlap this one
   G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
   if (b->n_core > 0 && b->pred_fill_needed) {
@@ -1467,170 +1823,127 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   int64_t visited = root_in ? 1 : 0;  // vertices with edges reached so far
   int64_t after_snap = 0;             // of them, reached after the gather's snapshot
   alg += 8 * words;                   // init writes the bitmap
+
+  HostLoop hl;
+  hl.n_c = n_c;
+  hl.m = g->m;
+  hl.alpha_thr = (int64_t)((double)g->m / alpha);
+  hl.beta_thr = (int64_t)((double)n_c / (beta * edgefactor * 2.0));
+  hl.exec_thr = b->exec_alpha > 0 ? (int64_t)std::floor((double)g->m / b->exec_alpha)
+                                  : INT64_MAX;
+  const uint32_t first = step_code(root_level + 1, kModeTDS);
+  LoopInit li{first, root_in ? 1 : 0, 0, hl.alpha_thr, hl.beta_thr, hl.exec_thr};
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1,
                      b->pred_new.get(),
                      (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get(),
-                     b->lsync.get());
+                     b->lsync.get(), li, g->row_off.get());
   G500_CHECK_LAUNCH();
 
   int64_t qlen = root_in ? 1 : 0;
-  int64_t est_edges = -1;  // unknown for the root: use a mid-size grid
-
-  // level-loop state (mpi/bfs.hpp:3264-3279)
-  bool next_td = true, growing = true;
-  int64_t prev_nq = 0;
-  int level = root_level;
-  const int64_t alpha_thr = (int64_t)((double)g->m / alpha);
-  const int64_t beta_thr = (int64_t)((double)n_c / (beta * edgefactor * 2.0));
-  LevelSync* ls = b->lsync.get();
-  unsigned long long rep[4];
-  bool unv_ok = false;  // unvisited lists match the visited bitmap
+  int64_t est_edges = -1;  // degree sum of the frontier (unknown for the root)
   bool early_gather = false;  // pred gather already running on gstream
-
+  int64_t last_nq = 0;
+  unsigned long long rep[4];
+  size_t nlev_ev = 0, nsweep_ev = 0;
+  std::deque<Step> pend;
+  std::vector<Step> ran;  // steps that ran, in order
+  int mispredicted = 0;
+  pend.push_back(launch_step(b, first, Bufs{b->vis.get(), b->vis_alt.get(), b->q.get(),
+                                            b->nq.get()},
+                             &nlev_ev, &nsweep_ev));
+  Bufs cur{};
   for (;;) {
-    at[G500_AT_LEVEL] = ++level;
-    const bool td = next_td;
-    hipEvent_t lev0 = nullptr, lev1 = nullptr;
-    if (b->timing) {
-      const size_t k = b->trace.size() * 2;
-      while (b->level_events.size() < k + 2) {
-        hipEvent_t e;
-        G500_CHECK_HIP(hipEventCreate(&e));
-        b->level_events.push_back(e);
-      }
-      lev0 = b->level_events[k];
-      lev1 = b->level_events[k + 1];
-      G500_CHECK_HIP(hipEventRecord(lev0, s));
-    }
-    const uint64_t level_bits = (uint64_t)level << 48;
-    unsigned long long seq = ++b->seq;
-    // a top-down level with a very large frontier runs as a bottom-up sweep
-    // followed by the bitmap -> queue conversion (same vertices and depths)
-    const bool exec_bu =
-        !td || (b->exec_alpha > 0 && est_edges > 0 &&
-                (double)est_edges > (double)g->m / b->exec_alpha);
-    const bool lists_before = unv_ok;  // a bottom-up level walks the lists
-    if (td && exec_bu) {
-      launch_bu(b, false, seq, level_bits, false);
-      unv_ok = true;
-      seq = ++b->seq;
-      launch_to_queue(b, b->nq.get(), seq, nullptr);
-    } else if (td) {
-      // small frontiers: the expansion stages them itself
-      if (qlen > kTdWin) {
-        const dim3 pg((unsigned)ceil_div(qlen, kPrepTile));
-        hipLaunchKernelGGL(k_td_prep_sums, pg, dim3(kT), 0, s, b->q.get(), qlen,
-                           g->row_off.get(), b->dsum.get(), b->bsum.get(), b->boff.get(),
-                           b->prep_ticket.get());
-        G500_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_td_prep_tiles, pg, dim3(kT), 0, s, qlen, b->dsum.get(),
-                           b->bsum.get(), b->boff.get(), b->tile_owner.get());
-        G500_CHECK_LAUNCH();
-      }
-      const unsigned grid = est_edges < 0 ? 256u : grid_for(est_edges, kTdTile);
-      hipLaunchKernelGGL(k_td_expand, dim3(grid), dim3(kT), 0, s,
-                         b->q.get(), qlen, b->dsum.get(), b->tile_owner.get(), g->row_off.get(),
-                         g->col.get(), g->new2orig.get(), g->deg_new.get(),
-                         (unsigned long long*)b->vis.get(), b->pred_new.get(), b->nq.get(),
-                         ls, b->lpart.get(), level_bits);
-      G500_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_level_report, dim3(1), dim3(kT), 0, s, (const uint64_t*)nullptr, 0,
-                         b->lpart.get(), (int)grid, ls, b->d_report, seq);
-      G500_CHECK_LAUNCH();
-      unv_ok = false;
+    if (b->spec_depth > 0) speculate(b, pend, hl, (size_t)b->spec_depth + 1, &nlev_ev, &nsweep_ev);
+    const Step st = pend.front();
+    pend.pop_front();
+    wait_report(b, st.seq, rep);
+    ran.push_back(st);
+    cur = st.after;
+    const uint32_t mode = step_mode(st.code);
+    uint32_t next;
+    if (mode == kModeConv) {
+      // the frontier of the first top-down level after the switch
+      qlen = (int64_t)rep[0];
+      est_edges = (int64_t)rep[1];
+      if (qlen != last_nq)
+        fail(G500_ERR_STATE, "frontier conversion mismatch (%lld vs %lld)", (long long)qlen,
+             (long long)last_nq);
+      alg += 24 * words + 8 * qlen;  // bitmap -> queue with the snapshot
+      next = step_code(step_level(st.code), td_step_mode(qlen, est_edges, hl.exec_thr));
     } else {
-      launch_bu(b, unv_ok, seq, level_bits, true);
-      unv_ok = true;
-    }
-    if (b->timing) G500_CHECK_HIP(hipEventRecord(lev1, s));
-    wait_report(b, seq, rep);
-    const int64_t nq_size = (int64_t)rep[0];
-    const int64_t nq_edges = td ? (int64_t)rep[1] : 0;
-    g500_level_trace tr{};
-    tr.level = level;
-    tr.top_down = td ? 1 : 0;
-    tr.nq_size = nq_size;
-    tr.nq_edges = nq_edges;
-    tr.exec_bottom_up = exec_bu ? 1 : 0;
-    b->trace.push_back(tr);
-    {
-      // bytes this level needs beyond the bottom-up kernels' counted scans
-      const int64_t unvisited = n_c - visited;
-      if ((td && exec_bu) || (!td && !lists_before)) {
-        // sweep: old bitmap, multi bitmap and new bitmap; a first-neighbour
-        // pair per unvisited vertex
-        const int64_t sw = 24 * words + 8 * unvisited;
-        sweep_host += sw;
-        alg += sw;
-        if (td) alg += 16 * words + 8 * nq_size;  // bitmap -> queue (queue + degree)
-      } else if (!td) {
-        // list level: old and new bitmap; a list entry and a pair per listed vertex
-        alg += 16 * words + 12 * unvisited;
-      } else {
-        // top-down: queue entry, row offsets and original ID per frontier
-        // vertex, one column entry per frontier edge, and per vertex found its
-        // entry, queue slot and degree (plus the prefix sums of a large frontier)
-        const int64_t fe = est_edges < 0 ? root_deg : est_edges;
-        alg += 24 * qlen + 4 * fe + 16 * nq_size + (qlen > kTdWin ? 16 * qlen : 0);
-      }
-      visited += nq_size;
-      if (early_gather) after_snap += nq_size;
-    }
-    if (nq_size == 0) break;
-
-    // direction decision (mpi/bfs.hpp:3413-3457)
-    if (growing && nq_size > prev_nq) {
-      if (td) {
-        if (nq_edges > alpha_thr) {
-          next_td = false;
-          at[G500_AT_GLOBAL_NQ_EDGES] = nq_edges;
-          at[G500_AT_NUM_GLOBAL_EDGES] = g->m;
-          at[G500_AT_T2B_LEVEL] = level;
+      const int level = step_level(st.code);
+      const bool td = mode_is_td(mode);
+      at[G500_AT_LEVEL] = level;
+      const int64_t nq_size = (int64_t)rep[0];
+      const int64_t nq_edges = td ? (int64_t)rep[1] : 0;
+      g500_level_trace tr{};
+      tr.level = level;
+      tr.top_down = td ? 1 : 0;
+      tr.nq_size = nq_size;
+      tr.nq_edges = nq_edges;
+      tr.exec_bottom_up = (mode == kModeTD || mode == kModeTDS) ? 0 : 1;
+      b->trace.push_back(tr);
+      {
+        // bytes this level needs beyond the bottom-up kernels' counted scans
+        const int64_t unvisited = n_c - visited;
+        if (mode == kModeTDX || mode == kModeBUSweep) {
+          // sweep: old bitmap, multi bitmap and new bitmap; a first-neighbour
+          // pair per unvisited vertex
+          const int64_t sw = 24 * words + 8 * unvisited;
+          sweep_host += sw;
+          alg += sw;
+          if (td) alg += 16 * words + 8 * nq_size;  // bitmap -> queue (queue + degree)
+        } else if (mode == kModeBUList) {
+          // list level: old and new bitmap; a list entry and a pair per listed vertex
+          alg += 16 * words + 12 * unvisited;
         } else {
-          at[G500_AT_PRE_GLOBAL_NQ_EDGES] = nq_edges;
+          // top-down: queue entry, row offsets and original ID per frontier
+          // vertex, one column entry per frontier edge, and per vertex found its
+          // entry, queue slot and degree (plus the prefix sums of a large frontier)
+          const int64_t fe = est_edges < 0 ? root_deg : est_edges;
+          alg += 24 * qlen + 4 * fe + 16 * nq_size + (qlen > kTdWin ? 16 * qlen : 0);
         }
+        visited += nq_size;
+        if (early_gather) after_snap += nq_size;
       }
-    } else if (!td) {
-      if (nq_size < beta_thr) {
-        next_td = true;
-        growing = false;
-        at[G500_AT_GLOBAL_NQ_SIZE] = nq_size;
-        at[G500_AT_NUM_GLOBAL_VERTS] = n_c;
-        at[G500_AT_B2T_LEVEL] = level;
-      } else {
-        at[G500_AT_PRE_GLOBAL_NQ_SIZE] = nq_size;
+      next = hl.after_level(st.code, nq_size, nq_edges, at);
+      last_nq = nq_size;
+      if (td && nq_size > 0) {
+        qlen = nq_size;
+        est_edges = nq_edges;
       }
     }
-
-    if (td) {
-      std::swap(b->q.p, b->nq.p);
-      qlen = nq_size;
-      est_edges = nq_edges;
-    } else if (next_td) {
-      // switching to top-down: frontier = bits new in this level.  The
-      // remaining top-down levels are short and latency-bound, so the pred
-      // gather of every vertex reached so far starts on the second stream
-      // once the conversion (which also snapshots the bitmap) is done; the
-      // vertices the last levels reach are patched in at the end
+    if ((uint32_t)rep[2] != next)
+      fail(G500_ERR_STATE, "level %d: the device chose step %u:%u, the reference rule %u:%u",
+           step_level(st.code), step_level((uint32_t)rep[2]), step_mode((uint32_t)rep[2]),
+           step_level(next), step_mode(next));
+    if (step_mode(next) == kModeDone) break;
+    if (pend.empty() || pend.front().code != next) {
+      // predicted wrong: the queued steps return at once
+      mispredicted += (int)pend.size();
+      pend.clear();
+      pend.push_back(launch_step(b, next, cur, &nlev_ev, &nsweep_ev));
+    }
+    if (step_mode(next) == kModeConv) {
+      // switching to top-down: the remaining top-down levels are short and
+      // latency-bound, so the pred gather of every vertex reached so far
+      // starts on the second stream once the conversion has written the
+      // bitmap snapshot (ev_snap, recorded when the conversion was queued);
+      // the vertices the last levels reach are patched in at the end
       // (k_pred_fixup)
-      const unsigned long long cseq = ++b->seq;
-      alg += 24 * words + 8 * nq_size;  // bitmap -> queue with the snapshot
-      launch_to_queue(b, b->q.get(), cseq, b->vis_snap.get());
       G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
       launch_gather(b, root, b->vis_snap.get(), b->gstream);
       G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
       early_gather = true;
-      wait_report(b, cseq, rep);
-      qlen = (int64_t)rep[0];
-      est_edges = (int64_t)rep[1];
-      if (qlen != nq_size)
-        fail(G500_ERR_STATE, "frontier conversion mismatch (%lld vs %lld)",
-             (long long)qlen, (long long)nq_size);
     }
-    prev_nq = nq_size;
   }
+  // the steps still queued (guesses past the last level) return at once
+  b->vis.p = cur.vis;
+  b->vis_alt.p = cur.vis_alt;
+  b->q.p = cur.q;
+  b->nq.p = cur.nq;
   if (early_gather) {
     G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
     hipLaunchKernelGGL(k_pred_fixup, dim3((unsigned)ceil_div(words, kT)), dim3(kT), 0, s,
@@ -1674,13 +1987,16 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_begin, b->ev_end));
   b->stats.gpu_ms = ms;
   b->stats.nlevels = (int)b->trace.size();
+  b->stats.mispredicted = mispredicted;
   G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_g0, b->ev_g1));
   b->stats.gather_ms = ms;
-  for (int i = 0; i < b->nsweeps; ++i) {
-    G500_CHECK_HIP(hipEventElapsedTime(&ms, b->sweep_events[2 * i], b->sweep_events[2 * i + 1]));
+  for (const Step& st : ran) {
+    if (st.sweep_ev < 0) continue;
+    G500_CHECK_HIP(hipEventElapsedTime(&ms, b->sweep_events[st.sweep_ev],
+                                       b->sweep_events[st.sweep_ev + 1]));
     b->stats.sweep_ms += ms;
+    ++b->stats.sweeps;
   }
-  b->stats.sweeps = b->nsweeps;
   if (b->counting) {
     std::vector<uint64_t> w(b->wacc.n);
     G500_CHECK_HIP(hipMemcpy(w.data(), b->wacc.get(), w.size() * sizeof(uint64_t),
@@ -1689,19 +2005,26 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     for (size_t i = 0; i < w.size(); ++i) (i < b->bu_grid[0] ? sweep_dev : other_dev) += w[i];
     b->stats.counted = 1;
     b->stats.alg_bytes = alg + (int64_t)(sweep_dev + other_dev);
-    b->stats.sweep_alg_bytes = b->nsweeps ? sweep_host + (int64_t)sweep_dev : 0;
+    b->stats.sweep_alg_bytes = b->stats.sweeps ? sweep_host + (int64_t)sweep_dev : 0;
     b->stats.gather_alg_bytes = gather_bytes;
   }
   if (b->timing) {
-    for (size_t i = 0; i < b->trace.size(); ++i) {
+    // per level: its step's device time; per boundary: the time from the end
+    // of one step that ran to the start of the next (host decisions, launch
+    // latency and any guessed step that returned at once)
+    size_t li_ = 0;
+    for (size_t i = 0; i < ran.size(); ++i) {
       float lm = 0;
-      G500_CHECK_HIP(hipEventElapsedTime(&lm, b->level_events[2 * i],
-                                         b->level_events[2 * i + 1]));
-      b->trace[i].gpu_ms = lm;
-      (b->trace[i].top_down ? b->stats.td_ms : b->stats.bu_ms) += lm;
+      G500_CHECK_HIP(hipEventElapsedTime(&lm, b->level_events[ran[i].ev],
+                                         b->level_events[ran[i].ev + 1]));
+      if (step_mode(ran[i].code) != kModeConv && li_ < b->trace.size()) {
+        b->trace[li_].gpu_ms = lm;
+        (b->trace[li_].top_down ? b->stats.td_ms : b->stats.bu_ms) += lm;
+        ++li_;
+      }
       if (i > 0) {
-        G500_CHECK_HIP(hipEventElapsedTime(&lm, b->level_events[2 * i - 1],
-                                           b->level_events[2 * i]));
+        G500_CHECK_HIP(hipEventElapsedTime(&lm, b->level_events[ran[i - 1].ev + 1],
+                                           b->level_events[ran[i].ev]));
         b->stats.gap_ms += lm;
         ++b->stats.gaps;
       }

@@ -5,19 +5,58 @@
 
 #include "internal.hpp"
 
-// Device-side counters that queue reservations take within a level
-// (top-down next-queue slots in cnt[0], long-row slots in cnt[2]).  The
-// level's one-block report kernel adds them to the blocks' totals it posts
-// and clears them; the per-root init clears them too.
+// Steps of the level loop.  A step is one level (top-down, a top-down level
+// run as a bottom-up sweep plus bitmap -> queue, or bottom-up over the bitmap
+// or the unvisited lists) or the bitmap -> queue conversion at the
+// bottom-up -> top-down switch.  Its code packs the level number and the mode.
+enum StepMode : uint32_t {
+  kModeTD = 1,       // top-down, frontier prepared by the prefix-sum kernels
+  kModeTDX = 2,      // top-down level run as a bottom-up sweep + bitmap -> queue
+  kModeBUSweep = 3,  // bottom-up over the bitmap
+  kModeBUList = 4,   // bottom-up over the unvisited lists
+  kModeConv = 5,     // bitmap -> queue at the bottom-up -> top-down switch
+  kModeDone = 6,
+  kModeTDS = 7,      // top-down, small frontier staged by the expansion itself
+};
+__host__ __device__ inline bool mode_is_td(uint32_t m) {
+  return m == kModeTD || m == kModeTDS || m == kModeTDX;
+}
+__host__ __device__ inline uint32_t step_code(int level, uint32_t mode) {
+  return ((uint32_t)level << 4) | mode;
+}
+__host__ __device__ inline uint32_t step_mode(uint32_t code) { return code & 15u; }
+__host__ __device__ inline int step_level(uint32_t code) { return (int)(code >> 4); }
+
+// Device-side state of the level loop.  cnt[] are the counters that queue
+// reservations take within a level (top-down next-queue slots in cnt[0],
+// long-row slots in cnt[2]); the step's closing kernel adds them to the
+// blocks' totals and clears them.  The closing kernel also applies the
+// direction rule and writes the code of the step that must run next into
+// `step`; every kernel of a step compares that code with its own and returns
+// at once when they differ.  So the host may queue a predicted step behind
+// the running one: a wrong prediction costs its empty launches, never a
+// result.  The per-root init kernel sets every field.
 struct LevelSync {
   unsigned long long cnt[4];
+  uint32_t step;       // code of the step to run next
+  uint32_t growing;    // the frontier has grown at every level so far
+  long long qlen;      // top-down frontier queue length
+  long long qedges;    // its degree sum (>= its CSR entries)
+  long long prev_nq;   // frontier size of the previous level
+  long long alpha_thr, beta_thr, exec_thr;  // int64(m/alpha), int64(n_c/(beta*ef*2)), m/exec_alpha
 };
 
-// Host-visible (pinned, coherent, mapped) level report.
+// Host-visible (pinned, coherent, mapped) step reports: a ring of slots, the
+// step with sequence number s posting to slot s % kReportSlots, so a step
+// queued ahead of the host never overwrites a report the host has not read.
+// Each word carries its value above a 16-bit tag (the low bits of s), so a
+// report is whole once every word carries the tag: no word needs to wait for
+// another (no system-scope fence on the device).
 struct HostReport {
-  unsigned long long seq;
-  unsigned long long v[4];
+  unsigned long long w[4];  // frontier size, frontier degree sum, next step code, 0 (<< 16 | tag)
 };
+constexpr int kReportSlots = 8;
+constexpr int kReportTagBits = 16;
 
 struct g500_bfs {
   g500_graph* g = nullptr;
@@ -36,10 +75,13 @@ struct g500_bfs {
   // bitmap words, unv_cnt[chunk] of them in use
   g500::DevBuf<uint32_t> unv, unv_cnt;
   unsigned bu_grid[3] = {1, 1, 1};  // resident blocks of the sweep / list / long-row kernels
-  g500::DevBuf<LevelSync> lsync;   // per-level counters
-  HostReport* h_report = nullptr;  // pinned host memory, written by the GPU
-  HostReport* d_report = nullptr;  // its device address
+  g500::DevBuf<LevelSync> lsync;   // level-loop state on the device
+  HostReport* h_report = nullptr;  // kReportSlots slots of pinned host memory, written by the GPU
+  HostReport* d_report = nullptr;  // their device address
   unsigned long long seq = 0;      // last report sequence number issued
+  // guessed steps queued behind the running one (G500_SPECULATE=<n>; 0: the
+  // host waits for every step's report before it launches the next)
+  int spec_depth = 1;
 
   // validation scratch
   g500::DevBuf<unsigned long long> vcount;  // [G500_VERR_NUM + 2]
@@ -61,13 +103,12 @@ struct g500_bfs {
   int64_t n_core = 0;
   int64_t iso_root = -1;  // a root without edges whose entry the last run wrote
   bool pred_fill_needed = false;  // pred must be reset to -1 before the next run
-  std::vector<hipEvent_t> level_events;  // 2 per level when timing is on
+  std::vector<hipEvent_t> level_events;  // 2 per launched step when timing is on
   bool timing = false;
   // per-kernel timing (always on): the gather's events on the stream it runs
   // on, and a pair around each k_bu_sweep launch of a run
   hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;
-  std::vector<hipEvent_t> sweep_events;
-  int nsweeps = 0;
+  std::vector<hipEvent_t> sweep_events;  // 2 per launched sweep
   // counting runs: per-block row-scan bytes of the bottom-up kernels, one
   // slot per block (sweep, long-row and list kernels in that order)
   bool counting = false;

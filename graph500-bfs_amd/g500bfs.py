@@ -48,7 +48,7 @@ class RunStats(C.Structure):
                 ("sweeps", C.c_int32), ("counted", C.c_int32),
                 ("alg_bytes", C.c_int64), ("sweep_alg_bytes", C.c_int64),
                 ("gather_alg_bytes", C.c_int64), ("gap_ms", C.c_double),
-                ("gaps", C.c_int32), ("reserved", C.c_int32)]
+                ("gaps", C.c_int32), ("mispredicted", C.c_int32)]
 
 
 class MsbfsTuning(C.Structure):

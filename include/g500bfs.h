@@ -234,11 +234,15 @@ typedef struct g500_run_stats {
   int64_t sweep_alg_bytes;
   int64_t gather_alg_bytes;
   /* with level timing on (g500_bfs_set_timing): device time between the end
-   * of one level's kernels and the start of the next level's, summed over
-   * the run's level boundaries (the host's per-level decision and launch) */
+   * of one step's kernels (a level, or the bitmap -> queue conversion at the
+   * bottom-up -> top-down switch) and the start of the next one's, summed
+   * over the run's step boundaries */
   double gap_ms;
-  int32_t gaps;         /* level boundaries summed in gap_ms */
-  int32_t reserved;
+  int32_t gaps;         /* step boundaries summed in gap_ms */
+  /* steps queued on a predicted direction that the device's decision did
+   * not take (each returned at once; the right step was queued after the
+   * level's report) */
+  int32_t mispredicted;
 } g500_run_stats;
 
 /* Enables per-level HIP-event timing (off by default; costs one event pair
