@@ -421,8 +421,7 @@ def main():
     kern = {
         "k_pred_gather_core": dict(ms=gather_ms, launches=np.ones(runs), bytes=gather_b,
                                    per_unit="24 B per vertex with edges (its original ID, "
-                                            "new ID, entry read and written) + the visited "
-                                            "bitmap"),
+                                            "new ID, entry read and written)"),
         "k_bu_sweep": dict(ms=sweep_ms, launches=sweeps.astype(float), bytes=sweep_b,
                            per_unit="per launch: old, multi and new bitmap (24 B per 64 "
                                     "vertices), an 8-B first-neighbour pair per unvisited "

@@ -73,6 +73,10 @@ constexpr int kTdItems = 8;
 constexpr int kTdTile = kT * kTdItems;
 constexpr int kTdWin = 512;  // owners staged in LDS per tile (more: read from memory)
 constexpr int kMaxGrid = 2048;
+// bits 32-47 of a level entry: the run's epoch (see run_entry); kEpochNone
+// marks entries of no run
+constexpr uint64_t kEpochMask = 0xFFFFull << 32;
+constexpr uint32_t kEpochNone = 0xFFFF;
 
 // The top-down step for a frontier of qlen vertices with degree sum qedges:
 // run as a sweep plus conversion past the execution threshold, else with or
@@ -1230,8 +1234,17 @@ __device__ inline XcdSpan xcd_span(int64_t n) {
   return x;
 }
 
+// Level entries carry the run's epoch in bits 32-47 (a parent's original ID
+// fits 32 bits): an entry is this run's iff its epoch matches, so the
+// gathers need no visited-bitmap probe, and pred gets the entry with the
+// epoch cleared.
+__device__ inline int64_t run_entry(int64_t e, uint32_t epoch) {
+  return (((uint64_t)e >> 32) & 0xFFFFull) == epoch ? (int64_t)((uint64_t)e & ~kEpochMask)
+                                                     : (int64_t)-1;
+}
+
 __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__ o2n,
-                                                   const uint64_t* __restrict__ vis,
+                                                   uint32_t epoch,
                                                    const int64_t* __restrict__ pred_new,
                                                    int64_t n, int64_t n_c, int64_t root,
                                                    const uint32_t* __restrict__ tree_parent,
@@ -1240,10 +1253,8 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
   const XcdSpan xs = xcd_span(n);
   for (int64_t o = xs.lo + xs.rank * kT + threadIdx.x; o < xs.hi; o += xs.nblk * kT) {
     const uint32_t v = o2n[o];
-    int64_t e = -1;
-    if ((int64_t)v < n_c && ((vis[v >> 6] >> (v & 63)) & 1ull)) {
-      e = pred_new[v];
-    } else if (has_parent && ((has_parent[o >> 6] >> (o & 63)) & 1ull)) {
+    int64_t e = (int64_t)v < n_c ? run_entry(pred_new[v], epoch) : (int64_t)-1;
+    if (e == -1 && has_parent && ((has_parent[o >> 6] >> (o & 63)) & 1ull)) {
       const uint32_t p = tree_parent[o];
       if (p != kNoParent) e = (int64_t)((uint64_t)p | (0xFFFFull << 48));
     }
@@ -1255,20 +1266,19 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
 // Without CoreBFS, a vertex without edges is never reached and its pred
 // entry is always -1: the engine fills pred once and the gather walks only
 // core, the (original ID, new ID) pairs of the vertices with edges in
-// ascending original-ID order.
+// ascending original-ID order.  One pair per thread: per vertex one
+// random 8-byte read (its entry), the rest streams.
 __global__ __launch_bounds__(kT) void k_pred_gather_core(const uint2* __restrict__ core,
-                                                        int64_t n_core,
-                                                        const uint64_t* __restrict__ vis,
+                                                        int64_t n_core, uint32_t epoch,
                                                         const int64_t* __restrict__ pred_new,
                                                         int64_t root,
                                                         int64_t* __restrict__ pred) {
   const XcdSpan xs = xcd_span(n_core);
   for (int64_t i = xs.lo + xs.rank * kT + threadIdx.x; i < xs.hi; i += xs.nblk * kT) {
     const uint2 ov = get_pair(core + i);
-    const uint32_t o = ov.x, v = ov.y;
-    int64_t e = ((vis[v >> 6] >> (v & 63)) & 1ull) ? pred_new[v] : -1;
-    if ((int64_t)o == root) e = root;
-    __builtin_nontemporal_store(e, pred + o);
+    int64_t e = run_entry(pred_new[ov.y], epoch);
+    if ((int64_t)ov.x == root) e = root;
+    __builtin_nontemporal_store(e, pred + ov.x);
   }
 }
 
@@ -1302,7 +1312,7 @@ __global__ __launch_bounds__(kT) void k_pred_fixup(const uint64_t* __restrict__ 
     const int b = __ffsll((long long)bits) - 1;
     bits &= bits - 1;
     const uint32_t v = (uint32_t)(w * 64 + b);
-    pred[n2o[v]] = pred_new[v];
+    pred[n2o[v]] = (int64_t)((uint64_t)pred_new[v] & ~kEpochMask);
   }
 }
 
@@ -1337,6 +1347,9 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   b->words = ceil_div(std::max<int64_t>(g->n_c, 1), 64);
   b->pred.alloc((size_t)g->n);
   b->pred_new.alloc((size_t)std::max<int64_t>(g->n_c, 1));
+  // every entry starts with the epoch no run uses (run_entry)
+  G500_CHECK_HIP(hipMemset(b->pred_new.get(), 0xFF, b->pred_new.n * sizeof(int64_t)));
+  b->epoch = 0xFFFFFFFFu;  // the first run's epoch is 0
   b->vis.alloc((size_t)b->words);
   b->vis_alt.alloc((size_t)b->words);
   b->q.alloc((size_t)std::max<int64_t>(g->n_c, 1));
@@ -1461,10 +1474,10 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
   }
 }
 
-// pred by original ID from the entries of the vertices set in `vis` (a
-// smaller grid beside the last top-down levels measured slower: the random
-// reads need every wave the chip holds).
-void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t st) {
+// pred by original ID from this run's level entries (a smaller grid beside
+// the last top-down levels measured slower: the random reads need every
+// wave the chip holds).
+void launch_gather(g500_bfs* b, int64_t root, hipStream_t st) {
   g500_graph* g = b->g;
   G500_CHECK_HIP(hipEventRecord(b->ev_g0, st));
   // grids are a multiple of kXcds (xcd_span)
@@ -1473,11 +1486,12 @@ void launch_gather(g500_bfs* b, int64_t root, const uint64_t* vis, hipStream_t s
   };
   if (b->n_core > 0)
     hipLaunchKernelGGL(k_pred_gather_core, dim3(grid(b->n_core)), dim3(kT), 0, st,
-                       b->core.get(), b->n_core, vis, b->pred_new.get(), root, b->pred.get());
+                       b->core.get(), b->n_core, b->epoch, b->pred_new.get(), root,
+                       b->pred.get());
   else
     hipLaunchKernelGGL(k_pred_gather, dim3(grid(g->n)),
-                       dim3(kT), 0, st, g->orig2new.get(), vis, b->pred_new.get(), g->n, g->n_c,
-                       root, g->corebfs ? g->tree_parent.get() : nullptr,
+                       dim3(kT), 0, st, g->orig2new.get(), b->epoch, b->pred_new.get(), g->n,
+                       g->n_c, root, g->corebfs ? g->tree_parent.get() : nullptr,
                        g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
   G500_CHECK_LAUNCH();
   G500_CHECK_HIP(hipEventRecord(b->ev_g1, st));
@@ -1529,7 +1543,7 @@ Step launch_step(g500_bfs* b, uint32_t code, const Bufs& in, size_t* nlev_ev,
   st.seq = ++b->seq;
   st.after = in;
   const uint32_t mode = step_mode(code);
-  const uint64_t level_bits = (uint64_t)step_level(code) << 48;
+  const uint64_t level_bits = ((uint64_t)step_level(code) << 48) | ((uint64_t)b->epoch << 32);
   if (b->timing) {
     st.ev = (int)*nlev_ev;
     *nlev_ev += 2;
@@ -1798,6 +1812,13 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->trace.clear();
   b->stats = g500_run_stats{};
   b->have_result = false;
+  // this run's epoch; before the epochs wrap, every entry is set to the
+  // epoch no run uses, so no entry of an earlier run can match
+  if (++b->epoch >= kEpochNone) {
+    G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
+    G500_CHECK_HIP(hipMemsetAsync(b->pred_new.get(), 0xFF, b->pred_new.n * sizeof(int64_t), s));
+    b->epoch = 0;
+  }
   // a gather left running by a run that failed must not overlap this one
   G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
   if (b->n_core > 0 && b->pred_fill_needed) {
@@ -1837,7 +1858,9 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(words, kT)), dim3(kT), 0, s,
                      b->vis.get(), words, n_c, root_in ? root_new : (int64_t)-1,
                      b->pred_new.get(),
-                     (int64_t)(((uint64_t)root_level << 48) | (uint64_t)bfs_root), b->q.get(),
+                     (int64_t)(((uint64_t)root_level << 48) | ((uint64_t)b->epoch << 32) |
+                               (uint64_t)bfs_root),
+                     b->q.get(),
                      b->lsync.get(), li, g->row_off.get());
   G500_CHECK_LAUNCH();
 
@@ -1934,7 +1957,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       // the vertices the last levels reach are patched in at the end
       // (k_pred_fixup)
       G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
-      launch_gather(b, root, b->vis_snap.get(), b->gstream);
+      launch_gather(b, root, b->gstream);
       G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
       early_gather = true;
     }
@@ -1953,13 +1976,12 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     // both bitmaps; per vertex its new -> original ID, entry read and written
     alg += 16 * words + 20 * after_snap;
   } else {
-    launch_gather(b, root, b->vis.get(), s);
+    launch_gather(b, root, s);
   }
   // the gather: per vertex with edges its original ID, new ID, entry read and
-  // written, plus the bitmap; with CoreBFS every original ID (new ID and
-  // entry written) and the reached vertices' entries
-  const int64_t gather_bytes = b->n_core > 0 ? 24 * b->n_core + 8 * words
-                                             : 12 * g->n + 8 * visited + 8 * words;
+  // written; with CoreBFS every original ID (new ID and entry written) and
+  // every vertex with edges' entry
+  const int64_t gather_bytes = b->n_core > 0 ? 24 * b->n_core : 12 * g->n + 8 * n_c;
   alg += gather_bytes;
   if (b->n_core > 0) {
     // the gather skips vertices without edges: a root among them gets its

@@ -102,6 +102,7 @@ struct g500_bfs {
   g500::DevBuf<uint2> core;
   int64_t n_core = 0;
   int64_t iso_root = -1;  // a root without edges whose entry the last run wrote
+  uint32_t epoch = 0;     // the run's epoch in bits 32-47 of its pred_new entries
   bool pred_fill_needed = false;  // pred must be reset to -1 before the next run
   std::vector<hipEvent_t> level_events;  // 2 per launched step when timing is on
   bool timing = false;
