@@ -185,21 +185,29 @@ struct MsLevel {
   int64_t hot;             // dense_algo 2: vertices below this are pulled
   int dense_algo;          // dense levels: 1 streaming pull, 2 hub pull + push
   int L;
+  // split level: roots whose frontier degree sum is below split_thr push;
+  // lane_part holds per-block degree sums per lane, split_mask the lanes
+  // that push (written by k_ms_lane_mask before the level's pulls)
+  unsigned long long split_thr;  // 0: no split levels
+  unsigned long long* lane_part;
+  uint64_t* split_mask;
 };
 
-enum { kModeTopDown = 0, kModeStream = 1, kModePull = 2 };
+enum { kModeTopDown = 0, kModeStream = 1, kModePull = 2, kModeSplit = 3 };
 enum { kDenseStream = 1, kDenseHybrid = 2 };
 
 __device__ inline bool ms_active(const MsLevel& a) { return a.cnt[a.L] != 0; }
 __device__ inline bool ms_top_down(const MsLevel& a, int l) { return a.fe[l] < a.thr; }
 // top-down while the frontier is light; else a streaming pull while few
-// (root, vertex) pairs are reached yet, then a per-vertex pull
+// (root, vertex) pairs are reached yet, then a per-vertex pull -- split at
+// the first bottom-up level after top-down ones (light roots push)
 __device__ inline int ms_mode(const MsLevel& a, int l) {
   if (ms_top_down(a, l)) return kModeTopDown;
   unsigned long long reached = 0;
   for (int j = 0; j <= l; ++j) reached += a.cnt[j];
   const double pairs = (double)a.d_roots[0] * (double)a.n_c;
-  return (double)reached < a.stream_frac * pairs ? kModeStream : kModePull;
+  if ((double)reached < a.stream_frac * pairs) return kModeStream;
+  return (a.split_thr > 0 && l > 0 && ms_top_down(a, l - 1)) ? kModeSplit : kModePull;
 }
 __device__ inline bool in_bitmap(const uint64_t* bm, int64_t v) {
   return (bm[v >> 6] >> (v & 63)) & 1ull;
@@ -237,7 +245,9 @@ __device__ inline void ms_note_lanes(const MsLevel& a, uint64_t (&lanes)[K]) {
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_prep(MsLevel a, int64_t bm_words) {
   if (!ms_active(a)) return;
-  const bool clear = ms_mode(a, a.L) != kModePull;  // F_{L+1} filled by atomicOr
+  // F_{L+1} filled by atomicOr (the pulls store every vertex's word)
+  const int mode = ms_mode(a, a.L);
+  const bool clear = mode != kModePull && mode != kModeSplit;
   const bool fold = a.L > 0 && ms_top_down(a, a.L - 1);
   const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * kT;
@@ -645,12 +655,133 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push_heavy(MsLevel a,
   }
 }
 
+// Split level.  A root whose level-L frontier is light (its degree sum below
+// split_thr) would keep almost every vertex's pull scanning its whole row:
+// few vertices have a neighbour in that frontier, and a pull stops only
+// when every lane it lacks is covered.  Those roots push instead, along the
+// rows of their frontier vertices, and the pulls stop once the other lanes
+// are covered.
+//
+// Per-lane degree sums of F_L: each block adds the degrees of its frontier
+// vertices into LDS counters, one per lane, and stores them to its own
+// lane_part row (kLaneParts rows of 256).
+constexpr int kLaneParts = 512;
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_lane_fe(MsLevel a) {
+  __shared__ unsigned long long lfe[64 * K];
+  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
+  for (int i = threadIdx.x; i < 64 * K; i += kT) lfe[i] = 0;
+  __syncthreads();
+  for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < a.n_c;
+       v += (int64_t)gridDim.x * kT) {
+    if (!in_bitmap(a.fb, v)) continue;
+    uint64_t x[K];
+    load_lanes<K>(a.fl + v * K, x);
+    const unsigned long long d = a.deg[v];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      for (uint64_t b = x[k]; b; b &= b - 1)
+        atomicAdd(&lfe[64 * k + __ffsll((long long)b) - 1], d);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * K; i += kT)
+    a.lane_part[(int64_t)blockIdx.x * 256 + i] = lfe[i];
+}
+
+// One block: sums the lane_part rows per lane and sets split_mask to the
+// lanes present in F_L whose frontier degree sum is below split_thr.
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_lane_mask(MsLevel a, int parts) {
+  __shared__ unsigned long long mk[4];
+  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
+  const int t = threadIdx.x;
+  if (t < 4) mk[t] = 0;
+  __syncthreads();
+  if (t < 64 * K) {
+    unsigned long long sum = 0;
+    for (int p = 0; p < parts; ++p) sum += a.lane_part[(int64_t)p * 256 + t];
+    const bool present = (a.act_l[t >> 6] >> (t & 63)) & 1ull;
+    if (present && sum < a.split_thr) atomicOr(&mk[t >> 6], 1ull << (t & 63));
+  }
+  __syncthreads();
+  if (t < 4) a.split_mask[t] = mk[t];
+}
+
+// Push side of a split level: the frontier vertices of [vb, ve), G lanes
+// each, OR their pushing lanes into F_{L+1} along their rows (k_ms_commit
+// then masks with seen).
+template <int K, int G>
+__global__ __launch_bounds__(kT) void k_ms_split_push(MsLevel a, int64_t vb, int64_t ve) {
+  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
+  uint64_t pm[K];
+  bool anyp = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    pm[k] = a.split_mask[k];
+    anyp = anyp || pm[k] != 0;
+  }
+  if (!anyp) return;
+  const int lig = threadIdx.x % G;
+  const int64_t gid = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
+  const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
+  for (int64_t v = vb + gid; v < ve; v += ngroups) {
+    if (!in_bitmap(a.fb, v)) continue;
+    uint64_t f[K];
+    load_lanes<K>(a.fl + v * K, f);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      f[k] &= pm[k];
+      any = any || f[k] != 0;
+    }
+    if (!any) continue;
+    const int64_t end = a.row_off[v + 1];
+    for (int64_t e = a.row_off[v] + lig; e < end; e += G) hyb_push_one<K>(a, ld_col(a.col + e), f);
+  }
+}
+
+// The same for heavy rows, one wave per work item of `chunk` edges.
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_split_push_heavy(MsLevel a,
+                                                           const uint64_t* __restrict__ item_start,
+                                                           const uint32_t* __restrict__ item_vertex,
+                                                           int64_t n_items, int chunk) {
+  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
+  uint64_t pm[K];
+  bool anyp = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    pm[k] = a.split_mask[k];
+    anyp = anyp || pm[k] != 0;
+  }
+  if (!anyp) return;
+  const int lane = lane_id();
+  const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+  for (int64_t it = wid; it < n_items; it += nw) {
+    const int64_t v = item_vertex[it];
+    if (!in_bitmap(a.fb, v)) continue;
+    uint64_t f[K];
+    load_lanes<K>(a.fl + v * K, f);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      f[k] &= pm[k];
+      any = any || f[k] != 0;
+    }
+    if (!any) continue;
+    const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
+    const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
+    for (int64_t e = beg + lane; e < end; e += kWave) hyb_push_one<K>(a, ld_col(a.col + e), f);
+  }
+}
+
 // After a streaming level: F_{L+1} = gathered lanes & ~seen, seen |= F_{L+1},
 // frontier bitmap, counters.
 template <int K>
-__global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a) {
+__global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a, int want) {
   __shared__ uint64_t red[kWavesPerBlock];
-  if (!ms_active(a) || ms_mode(a, a.L) != kModeStream) return;
+  if (!ms_active(a) || ms_mode(a, a.L) != want) return;
   uint64_t disc = 0, edges = 0, lanes[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) lanes[k] = 0;
@@ -690,7 +821,12 @@ __global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a) {
 template <int K, int G, int U>
 __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve) {
   __shared__ uint64_t red[kWavesPerBlock];
-  if (!ms_active(a) || ms_mode(a, a.L) != kModePull) return;
+  if (!ms_active(a)) return;
+  const int mode = ms_mode(a, a.L);
+  if (mode != kModePull && mode != kModeSplit) return;
+  // split level: the pushing lanes count as covered too, and the raw lanes
+  // found go to F_{L+1} for k_ms_commit (which masks, counts and marks)
+  const bool split = mode == kModeSplit;
   const int lig = threadIdx.x % G;
   const int64_t gid0 = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
   const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
@@ -701,7 +837,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
   uint64_t inact[K], lanes[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    inact[k] = ~a.act_l[k];
+    inact[k] = ~a.act_l[k] | (split ? a.split_mask[k] : 0ull);
     lanes[k] = 0;
   }
   // the loop runs wave-uniformly (lanes past ve idle) so that the frontier
@@ -711,10 +847,13 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
   for (int64_t vbase = vb + wave_first; vbase < ve; vbase += ngroups) {
     const int64_t v = vbase + lane_id() / G;
     const bool valid = v < ve;
-    uint64_t nxt[K];
+    uint64_t nxt[K], acc[K];
     bool nz = false;
 #pragma unroll
-    for (int k = 0; k < K; ++k) nxt[k] = 0;
+    for (int k = 0; k < K; ++k) {
+      nxt[k] = 0;
+      acc[k] = 0;
+    }
     if (valid) {
       uint64_t s0[K], f[K], s[K];
       load_lanes<K>(a.seen + v * K, s0);
@@ -727,9 +866,6 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
       }
       if (!full) {
         const int64_t beg = a.row_off[v], end = a.row_off[v + 1];
-        uint64_t acc[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = 0;
         for (int64_t base = beg; base < end; base += (int64_t)G * U) {
           uint32_t u[U];
 #pragma unroll
@@ -757,13 +893,18 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
           for (int k = 0; k < K; ++k) cov = cov && ((acc[k] | s[k] | inact[k]) == ~0ull);
           if (cov) break;
         }
+        if (!split) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          nxt[k] = acc[k] & ~s[k];
-          nz = nz || nxt[k] != 0;
+          for (int k = 0; k < K; ++k) {
+            nxt[k] = acc[k] & ~s[k];
+            nz = nz || nxt[k] != 0;
+          }
         }
       }
-      if (lig == 0) {
+      if (lig == 0 && split) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) a.fn[v * K + k] = acc[k];
+      } else if (lig == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           a.fn[v * K + k] = nxt[k];
@@ -804,13 +945,18 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy(MsLevel a,
                                                    const uint32_t* __restrict__ item_vertex,
                                                    int64_t n_items, int chunk,
                                                    uint64_t* __restrict__ gacc) {
-  if (!ms_active(a) || ms_mode(a, a.L) != kModePull) return;
+  if (!ms_active(a)) return;
+  const int mode = ms_mode(a, a.L);
+  if (mode != kModePull && mode != kModeSplit) return;
   const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
-  uint64_t inact[K];  // lanes absent from F_L count as covered (as in k_ms_bu)
+  // lanes absent from F_L (and on a split level the pushing lanes) count as
+  // covered (as in k_ms_bu)
+  uint64_t inact[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) inact[k] = ~a.act_l[k];
+  for (int k = 0; k < K; ++k)
+    inact[k] = ~a.act_l[k] | (mode == kModeSplit ? a.split_mask[k] : 0ull);
   for (int64_t it = wid; it < n_items; it += nw) {
     const int64_t v = item_vertex[it];
     uint64_t s[K], g0[K], f[K];
@@ -868,7 +1014,23 @@ template <int K>
 __global__ __launch_bounds__(kT) void k_ms_bu_heavy_fin(MsLevel a, int64_t h,
                                                        uint64_t* __restrict__ gacc) {
   __shared__ uint64_t red[kWavesPerBlock];
-  if (!ms_active(a) || ms_mode(a, a.L) != kModePull) return;
+  if (!ms_active(a)) return;
+  const int mode = ms_mode(a, a.L);
+  if (mode == kModeSplit) {
+    // raw lanes to F_{L+1}; k_ms_commit masks, counts and marks them
+    for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < h;
+         v += (int64_t)gridDim.x * kT) {
+      uint64_t x[K];
+      load_lanes<K>(gacc + v * K, x);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        a.fn[v * K + k] = x[k];
+        if (x[k]) gacc[v * K + k] = 0;
+      }
+    }
+    return;
+  }
+  if (mode != kModePull) return;
   uint64_t disc = 0, edges = 0, lanes[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) lanes[k] = 0;
@@ -1197,6 +1359,9 @@ MsLevel level_args(const g500_msbfs* b, int L) {
   a.hot = 0;  // set per lane-word count by level_kernels
   a.n_c = g->n_c;
   a.L = L;
+  a.split_thr = (unsigned long long)((double)b->tuning.split_bp / 10000.0 * (double)g->m);
+  a.lane_part = b->lane_part.get();
+  a.split_mask = b->split_mask.get();
   return a;
 }
 
@@ -1204,6 +1369,15 @@ template <int K, int G>
 void launch_td(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve, hipStream_t s) {
   if (ve <= vb) return;
   hipLaunchKernelGGL((k_ms_td<K, G>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s, a, vb, ve);
+  G500_CHECK_LAUNCH();
+}
+
+template <int K, int G>
+void launch_split_push(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve,
+                       hipStream_t s) {
+  if (ve <= vb) return;
+  hipLaunchKernelGGL((k_ms_split_push<K, G>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s, a,
+                     vb, ve);
   G500_CHECK_LAUNCH();
 }
 
@@ -1301,9 +1475,15 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
       default: launch_hyb_push<K, 32>(b, a, h, m, s); break;
     }
     launch_hyb_push<K, 1>(b, a, m, n_c, s);
-    hipLaunchKernelGGL((k_ms_commit<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a);
+    hipLaunchKernelGGL((k_ms_commit<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a,
+                       (int)kModeStream);
     G500_CHECK_LAUNCH();
   }
+  // split level: per-lane frontier degree sums and the pushing lanes first
+  hipLaunchKernelGGL((k_ms_lane_fe<K>), dim3(kLaneParts), dim3(kT), 0, s, a);
+  G500_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_ms_lane_mask<K>), dim3(1), dim3(kT), 0, s, a, kLaneParts);
+  G500_CHECK_LAUNCH();
   if (b->n_items > 0) {
     hipLaunchKernelGGL((k_ms_bu_heavy<K, 4>), dim3(hgrid), dim3(kT), 0, s, a,
                        b->item_start.get(), b->item_vertex.get(), b->n_items, chunk,
@@ -1321,6 +1501,24 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
     default: launch_bu<K, 32, 2>(b, a, h, m, s); break;
   }
   launch_bu<K, 1, 4>(b, a, m, n_c, s);
+  // split level: the light roots push after the pulls stored F_{L+1}, then
+  // the commit masks, counts and marks
+  if (b->n_items > 0) {
+    hipLaunchKernelGGL((k_ms_split_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
+                       b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
+    G500_CHECK_LAUNCH();
+  }
+  switch (b->tuning.medium_group) {
+    case 2: launch_split_push<K, 2>(b, a, h, m, s); break;
+    case 4: launch_split_push<K, 4>(b, a, h, m, s); break;
+    case 8: launch_split_push<K, 8>(b, a, h, m, s); break;
+    case 16: launch_split_push<K, 16>(b, a, h, m, s); break;
+    default: launch_split_push<K, 32>(b, a, h, m, s); break;
+  }
+  launch_split_push<K, 1>(b, a, m, n_c, s);
+  hipLaunchKernelGGL((k_ms_commit<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a,
+                     (int)kModeSplit);
+  G500_CHECK_LAUNCH();
 }
 
 void enqueue_level(const g500_msbfs* b, int K, int L, hipStream_t s) {
@@ -1528,6 +1726,7 @@ int64_t msbfs_footprint(const g500_graph* g, int max_roots, bool level_arrays) {
   bytes += 2 * bw * 8;                                // frontier bitmaps
   bytes += 4 * (kMaxLevels + 2) * 8 + 8 * 1024;       // counters, roots, te, tables
   bytes += (ceil_div(g->m, kStreamUnit) + 3) * 4;     // streaming tile table
+  bytes += (int64_t)kLaneParts * 256 * 8 + 4 * 8;     // split level: lane sums, mask
   return bytes;
 }
 
@@ -1544,6 +1743,9 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
   b->fptr.alloc(kMaxLevels + 1);  // F_0..F_255
   b->cnt.alloc(kMaxLevels + 2);
   b->te.alloc(G500_MSBFS_MAX_ROOTS);
+  b->lane_part.alloc((size_t)kLaneParts * 256);
+  b->split_mask.alloc(4);
+  G500_CHECK_HIP(hipMemset(b->split_mask.get(), 0, 4 * sizeof(uint64_t)));
   b->roots.alloc(G500_MSBFS_MAX_ROOTS + 1);
   b->level_arrays = level_arrays;
   if (level_arrays) b->out.alloc((size_t)max_roots * (size_t)g->n);
@@ -1584,6 +1786,7 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
   t.stream_frac = 0.0;  // dense levels off until measured faster (tests force them)
   t.hot_bytes = 2 << 20;
   t.dense_algo = kDenseHybrid;
+  t.split_bp = 200;
   msbfs_set_tuning(b, t);
   G500_CHECK_HIP(hipStreamSynchronize(b->stream));
 }
@@ -1605,6 +1808,8 @@ void msbfs_set_tuning(g500_msbfs* b, const g500_msbfs_tuning& t) {
   if (t.dense_algo != kDenseStream && t.dense_algo != kDenseHybrid)
     fail(G500_ERR_ARG, "tuning: dense_algo is 1 (stream) or 2 (hub pull + push)");
   if (t.hot_bytes < 0) fail(G500_ERR_ARG, "tuning: hot_bytes must be >= 0");
+  if (t.split_bp < 0 || t.split_bp > 10000)
+    fail(G500_ERR_ARG, "tuning: split_bp must be in [0, 10000]");
   b->tuning = t;
   drop_graphs(b);
   update_bins(b);
@@ -1803,9 +2008,10 @@ double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int round
   const double sfrac[] = {0.0, 0.05, 0.2};
   const int64_t hotb[] = {1 << 20, 2 << 20, 4 << 20, 8 << 20, 32 << 20};
   const int32_t dalgo[] = {kDenseStream, kDenseHybrid};
+  const int32_t split[] = {0, 50, 100, 200, 400, 800};
   for (int round = 0; round < rounds; ++round) {
-    for (int p = 0; p < 9; ++p) {
-      static const int ncand[9] = {6, 6, 4, 4, 5, 7, 3, 5, 2};
+    for (int p = 0; p < 10; ++p) {
+      static const int ncand[10] = {6, 6, 4, 4, 5, 7, 3, 5, 2, 6};
       const int nc = ncand[p];
       for (int c = 0; c < nc; ++c) {
         g500_msbfs_tuning t = best;
@@ -1818,7 +2024,8 @@ double msbfs_autotune(g500_msbfs* b, const int64_t* roots, int nroots, int round
           case 5: t.td_alpha = alpha[c]; break;
           case 6: t.stream_frac = sfrac[c]; break;
           case 7: t.hot_bytes = hotb[c]; break;
-          default: t.dense_algo = dalgo[c]; break;
+          case 8: t.dense_algo = dalgo[c]; break;
+          default: t.split_bp = split[c]; break;
         }
         if (t.medium_degree > t.heavy_degree) continue;
         if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;

@@ -52,6 +52,8 @@ struct g500_msbfs {
   std::vector<uint64_t> h_item_start;       // host copy of item_start
   int64_t n_items = 0;
   g500::DevBuf<uint64_t> acc;               // heavy pull accumulators (h * kmax)
+  g500::DevBuf<unsigned long long> lane_part;  // split level: per-block lane degree sums
+  g500::DevBuf<uint64_t> split_mask;        // split level: the lanes that push (4 words)
   g500::ScanWorkspace scan_ws;
 
   // one instantiated graph per K (the level loop is captured once per K)

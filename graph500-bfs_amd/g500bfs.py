@@ -56,7 +56,7 @@ class MsbfsTuning(C.Structure):
                 ("medium_group", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("use_graph", C.c_int32), ("heavy_chunk", C.c_int32),
                 ("td_alpha", C.c_double), ("stream_frac", C.c_double),
-                ("hot_bytes", C.c_int64), ("dense_algo", C.c_int32), ("reserved", C.c_int32)]
+                ("hot_bytes", C.c_int64), ("dense_algo", C.c_int32), ("split_bp", C.c_int32)]
 
 
 class MsbfsStats(C.Structure):
@@ -367,7 +367,7 @@ class MultiBfs:
     def tuning(self):
         t = MsbfsTuning()
         _check(load().g500_msbfs_get_tuning(self.h, C.byref(t)))
-        return {f: getattr(t, f) for f, _ in MsbfsTuning._fields_ if f != "reserved"}
+        return {f: getattr(t, f) for f, _ in MsbfsTuning._fields_}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
@@ -423,7 +423,7 @@ class MultiBfs:
         _check(load().g500_msbfs_autotune(self.h, _p(r, C.c_int64), len(r), int(rounds),
                                           C.byref(t), C.byref(ms)))
         self.nroots = len(r)
-        return {f: getattr(t, f) for f, _ in MsbfsTuning._fields_ if f != "reserved"}, ms.value
+        return {f: getattr(t, f) for f, _ in MsbfsTuning._fields_}, ms.value
 
     def te(self):
         out = np.empty(max(self.nroots, 1), np.int64)

@@ -288,7 +288,9 @@ typedef struct g500_msbfs g500_msbfs;
  * balanced stream over every edge; 2 = every vertex pulls from its neighbours
  * below the hub cut-off (the lowest IDs, whose lane words take hot_bytes and
  * stay cached), and the frontier vertices above it push along their rows.
- * Later bottom-up levels pull per vertex with early exit.
+ * Later bottom-up levels pull per vertex with early exit; at the first of
+ * them the roots with light frontiers (split_bp) push along their frontier
+ * rows instead, so a vertex's pull stops once the other roots are covered.
  * blocks_per_cu sizes the grid of the persistent sweep kernels; use_graph = 1
  * replays the whole batch as one captured HIP graph. */
 typedef struct g500_msbfs_tuning {
@@ -302,7 +304,10 @@ typedef struct g500_msbfs_tuning {
   double stream_frac;    /* 0..1; 0 = no dense levels */
   int64_t hot_bytes;     /* dense_algo 2: lane words of the lowest IDs kept hot */
   int32_t dense_algo;    /* dense levels: 1 streaming pull, 2 hub pull + push */
-  int32_t reserved;      /* 0 */
+  int32_t split_bp;      /* 0..10000: at the first bottom-up level after top-down
+                            ones, the roots whose frontier holds fewer than
+                            split_bp / 10000 * m edges push instead of being
+                            pulled (0 = every root pulled) */
 } g500_msbfs_tuning;
 
 typedef struct g500_msbfs_stats {

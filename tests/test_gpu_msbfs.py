@@ -6,8 +6,10 @@ the plain GPU engine on larger ones.  Traversed-edge counts must equal the
 validator's TE for the same root.  Covered: 1..256 roots (ragged lane words,
 duplicate roots, roots without edges), every degree-bin layout, graph replay
 against eager launches, every direction mode (top-down, streaming pull,
-hub pull + push, per-vertex pull) forced on every level, the asynchronous launch/query/wait
-form, BFS depths beyond the captured level count, and the uint8 depth limit.
+hub pull + push, per-vertex pull) forced on every level, the split level (light
+roots push while the others pull) at every push threshold, the asynchronous
+launch/query/wait form, BFS depths beyond the captured level count, and the
+uint8 depth limit.
 """
 import numpy as np
 import pytest
@@ -145,12 +147,18 @@ def test_batch_with_roots_in_tiny_components(G, dense_algo):
     dict(td_alpha=4.0, stream_frac=0.3, dense_algo=2, hot_bytes=2048),
     dict(heavy_degree=16, medium_degree=4, heavy_chunk=256),
     dict(heavy_degree=16, medium_degree=4, heavy_chunk=65536, td_alpha=1.0),
+    dict(split_bp=200),                           # light roots push at the first pull level
+    dict(split_bp=10000),                         # every root pushes there
+    dict(split_bp=1),
+    dict(split_bp=500, heavy_degree=16, medium_degree=4, heavy_chunk=256, medium_group=4),
+    dict(split_bp=10000, td_alpha=100.0, heavy_degree=1, medium_degree=1),
 ])
 def test_bins_and_modes_do_not_change_levels(G, s12, tuning):
     print(f"\n[msbfs] tuning {tuning}", flush=True)
     g, og = s12
     roots = _mixed_roots(g, og, 100, seed=7)
     ref = G.MultiBfs(g, max_roots=128)
+    ref.set_tuning(split_bp=0)  # every bottom-up level a plain per-vertex pull
     ref.run(roots)
     mb = G.MultiBfs(g, max_roots=128)
     mb.set_tuning(**tuning)
