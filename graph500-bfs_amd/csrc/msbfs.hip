@@ -666,11 +666,15 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push_heavy(MsLevel a,
 // vertices into LDS counters, one per lane, and stores them to its own
 // lane_part row (kLaneParts rows of 256).
 constexpr int kLaneParts = 512;
+// A word with more lanes set than clear adds its degree to the word's
+// all-lanes counter and subtracts it from its clear lanes (modulo 2^64), so
+// no word costs more than 33 LDS atomics.
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_lane_fe(MsLevel a) {
-  __shared__ unsigned long long lfe[64 * K];
+  __shared__ unsigned long long lfe[64 * K], lall[K];
   if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
   for (int i = threadIdx.x; i < 64 * K; i += kT) lfe[i] = 0;
+  if (threadIdx.x < K) lall[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < a.n_c;
        v += (int64_t)gridDim.x * kT) {
@@ -679,27 +683,55 @@ __global__ __launch_bounds__(kT) void k_ms_lane_fe(MsLevel a) {
     load_lanes<K>(a.fl + v * K, x);
     const unsigned long long d = a.deg[v];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      for (uint64_t b = x[k]; b; b &= b - 1)
-        atomicAdd(&lfe[64 * k + __ffsll((long long)b) - 1], d);
+    for (int k = 0; k < K; ++k) {
+      if (__popcll(x[k]) > 32) {
+        atomicAdd(&lall[k], d);
+        for (uint64_t b = ~x[k]; b; b &= b - 1)
+          atomicAdd(&lfe[64 * k + __ffsll((long long)b) - 1], 0ull - d);
+      } else {
+        for (uint64_t b = x[k]; b; b &= b - 1)
+          atomicAdd(&lfe[64 * k + __ffsll((long long)b) - 1], d);
+      }
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 64 * K; i += kT)
-    a.lane_part[(int64_t)blockIdx.x * 256 + i] = lfe[i];
+    a.lane_part[(int64_t)blockIdx.x * 256 + i] = lfe[i] + lall[i / 64];
 }
 
-// One block: sums the lane_part rows per lane and sets split_mask to the
-// lanes present in F_L whose frontier degree sum is below split_thr.
+// One block: sums the lane_part rows per lane (each wave a quarter of the
+// rows, 16 loads in flight per lane) and sets split_mask to the lanes
+// present in F_L whose frontier degree sum is below split_thr.
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_lane_mask(MsLevel a, int parts) {
   __shared__ unsigned long long mk[4];
+  __shared__ unsigned long long ws[kWavesPerBlock][64 * K];
   if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, wave = t / kWave, lane = lane_id();
   if (t < 4) mk[t] = 0;
+  const int per = (parts + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int p0 = wave * per, p1 = min(parts, p0 + per);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int lanei = 64 * k + lane;
+    unsigned long long acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0;
+    for (int p = p0; p < p1; p += 16) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (p + j < p1) acc[j] += a.lane_part[(int64_t)(p + j) * 256 + lanei];
+    }
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sum += acc[j];
+    ws[wave][lanei] = sum;
+  }
   __syncthreads();
   if (t < 64 * K) {
     unsigned long long sum = 0;
-    for (int p = 0; p < parts; ++p) sum += a.lane_part[(int64_t)p * 256 + t];
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) sum += ws[w][t];
     const bool present = (a.act_l[t >> 6] >> (t & 63)) & 1ull;
     if (present && sum < a.split_thr) atomicOr(&mk[t >> 6], 1ull << (t & 63));
   }
