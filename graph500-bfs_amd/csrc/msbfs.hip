@@ -241,14 +241,16 @@ __device__ inline void ms_note_lanes(const MsLevel& a, uint64_t (&lanes)[K]) {
 
 // Level preparation: clear the next bitmap; for a top-down or streaming level
 // clear F_{L+1} (it is filled by atomicOr); when F_L came from a top-down
-// level, fold it into seen (the other modes write seen themselves).
+// level and this level pushes too, fold F_L into seen (a level that pulls
+// ORs F_L into each vertex's seen word itself, and the other modes write
+// seen themselves).
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_prep(MsLevel a, int64_t bm_words) {
   if (!ms_active(a)) return;
   // F_{L+1} filled by atomicOr (the pulls store every vertex's word)
   const int mode = ms_mode(a, a.L);
   const bool clear = mode != kModePull && mode != kModeSplit;
-  const bool fold = a.L > 0 && ms_top_down(a, a.L - 1);
+  const bool fold = a.L > 0 && ms_top_down(a, a.L - 1) && mode == kModeTopDown;
   const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * kT;
   for (int64_t i = i0; i < bm_words; i += stride) a.fbn[i] = 0;
@@ -1446,6 +1448,11 @@ template <int K>
 void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
   MsLevel a = level_args(b, L);
   a.hot = std::min<int64_t>(b->g->n_c, b->tuning.hot_bytes / (8 * K));
+  // with more than two lane words a pushed lane costs more atomics per
+  // edge and a pulled vertex's scan covers more lanes at once: the push
+  // threshold shrinks as 2/K (measured best at SCALE 26: 200 basis points
+  // for 64 and 128 roots, 50-100 for 256)
+  if (K > 2) a.split_thr = (unsigned long long)((double)a.split_thr * 2.0 / K);
   const int64_t h = b->bin_heavy, m = b->bin_medium, n_c = b->g->n_c;
   const int chunk = b->tuning.heavy_chunk;
   const int64_t bw = bitmap_words(b);

@@ -225,11 +225,34 @@ def run_open_loop(server, queries, clock=time.perf_counter):
 
 
 def latency_summary(queries):
-    lat = np.array([q.t_done - q.t_arrive for q in queries]) * 1e3
-    span = max(q.t_done for q in queries) - min(q.t_arrive for q in queries)
-    return {"queries": len(queries), "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-            "qps": len(queries) / span}
+    """Whole-run figures (qps = queries over the span from the first arrival
+    to the last completion, so the last batches' latency counts against it)
+    and steady-state ones: the queries arriving in the middle half of the
+    arrival period, their latency percentiles, the completion rate over that
+    window, and the mean latency of the first and the last quarter of
+    arrivals (a queue that grows shows as a rise between the two)."""
+    arr = np.array([q.t_arrive for q in queries])
+    done = np.array([q.t_done for q in queries])
+    lat = (done - arr) * 1e3
+    span = done.max() - arr.min()
+    out = {"queries": len(queries), "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+           "qps": len(queries) / span}
+    a0, a1 = np.percentile(arr, 25), np.percentile(arr, 75)
+    mid = (arr >= a0) & (arr < a1)
+    if a1 > a0 and mid.any():
+        order = np.argsort(arr)
+        q4 = max(1, len(order) // 4)
+        out.update({
+            "steady_window_s": float(a1 - a0),
+            "steady_arrival_qps": float(mid.sum() / (a1 - a0)),
+            "steady_completion_qps": float(((done >= a0) & (done < a1)).sum() / (a1 - a0)),
+            "steady_p50_ms": float(np.percentile(lat[mid], 50)),
+            "steady_p99_ms": float(np.percentile(lat[mid], 99)),
+            "first_quarter_mean_ms": float(lat[order[:q4]].mean()),
+            "last_quarter_mean_ms": float(lat[order[-q4:]].mean()),
+        })
+    return out
 
 
 def one_per_launch(graphs, queries, clock=time.perf_counter):

@@ -306,8 +306,9 @@ typedef struct g500_msbfs_tuning {
   int32_t dense_algo;    /* dense levels: 1 streaming pull, 2 hub pull + push */
   int32_t split_bp;      /* 0..10000: at the first bottom-up level after top-down
                             ones, the roots whose frontier holds fewer than
-                            split_bp / 10000 * m edges push instead of being
-                            pulled (0 = every root pulled) */
+                            split_bp / 10000 * m edges (times 2/K for K > 2
+                            lane words) push instead of being pulled
+                            (0 = every root pulled) */
 } g500_msbfs_tuning;
 
 typedef struct g500_msbfs_stats {

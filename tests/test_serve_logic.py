@@ -137,3 +137,25 @@ def test_poisson_workload_and_latency_summary():
         q.t_done = q.t_arrive + (i + 1) * 1e-3
     s = serve.latency_summary(qs[:100])
     assert s["p50_ms"] == pytest.approx(50.5, rel=0.02) and s["p99_ms"] > s["p50_ms"]
+
+
+def test_latency_summary_steady_state():
+    """Queries every 1 ms for 4 s with a constant 30 ms latency: the steady
+    window (middle half of the arrivals) completes 1000 q/s at 30 ms, the
+    first and last quarters have the same mean; a latency that grows with
+    arrival time shows as a rise from the first quarter to the last."""
+    qs = [serve.Query(i, 0, 0, [], t_arrive=i * 1e-3) for i in range(4000)]
+    for q in qs:
+        q.t_done = q.t_arrive + 0.030
+    s = serve.latency_summary(qs)
+    assert s["steady_window_s"] == pytest.approx(2.0, rel=0.01)
+    assert s["steady_arrival_qps"] == pytest.approx(1000.0, rel=0.01)
+    assert s["steady_completion_qps"] == pytest.approx(1000.0, rel=0.02)
+    assert s["steady_p50_ms"] == pytest.approx(30.0, rel=1e-6)
+    assert s["first_quarter_mean_ms"] == pytest.approx(s["last_quarter_mean_ms"], rel=1e-6)
+    assert s["qps"] < 1000.0  # the whole-run figure pays the last latency
+    for i, q in enumerate(qs):
+        q.t_done = q.t_arrive + 0.030 + i * 1e-5  # queue growing by 10 us per query
+    g = serve.latency_summary(qs)
+    assert g["last_quarter_mean_ms"] > g["first_quarter_mean_ms"] + 25.0
+    assert g["steady_completion_qps"] < g["steady_arrival_qps"]
