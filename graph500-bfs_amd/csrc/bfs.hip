@@ -1653,12 +1653,16 @@ struct HostLoop {
   int64_t prev_nq = 0;
   int64_t alpha_thr = 0, beta_thr = 0, exec_thr = 0;
   int64_t n_c = 0, m = 0;
+  // for the guesses: vertices with edges reached so far, and how many of
+  // them the engine's previous run left unreached (other components)
+  int64_t visited = 0, unreach = 0;
 
   // next step after level step `code` reported (nq_size, nq_edges)
   uint32_t after_level(uint32_t code, int64_t nq_size, int64_t nq_edges, int64_t* at) {
     const uint32_t mode = step_mode(code);
     const int level = step_level(code);
     const bool td = mode_is_td(mode);
+    visited += nq_size;
     if (nq_size == 0) return step_code(level + 1, kModeDone);
     bool next_td = td;
     if (growing && nq_size > prev_nq) {
@@ -1693,25 +1697,28 @@ struct HostLoop {
 };
 
 // Frontier totals (size, degree sum) that a level of mode `mode` is expected
-// to find, extrapolated from the last two levels of `tr` (reported or
-// themselves estimated): the growth from the level before the last one to
-// the last one, applied once more.
-void estimate_level(uint32_t mode, const std::vector<g500_level_trace>& tr, int64_t* nq,
-                    int64_t* fe) {
+// to find, from the last level of `tr` (reported or itself estimated).  A
+// top-down level over a frontier of degree sum fe0 finds about fe0 new
+// vertices (early on, most edges lead to unvisited ones), whose degree sum
+// on Kronecker graphs is about 24 sqrt(m fe0) (the frontier reaches the hubs
+// at once: fitted to SCALE-24 and 26 runs, within a factor of two from fe0
+// ~ 10^3 to 10^7).  A bottom-up level finds the last level's count times its
+// growth, but at most 0.8 of the vertices still unreached that are not in
+// another component (learned from the previous run).
+void estimate_level(uint32_t mode, const std::vector<g500_level_trace>& tr, const HostLoop& hl,
+                    int64_t* nq, int64_t* fe) {
   const size_t k = tr.size();
-  auto grow = [&](int64_t a, int64_t b, double dflt) {
-    return (a > 0 && b > 0) ? (double)a / (double)b : dflt;
-  };
-  const double cap = (double)(INT64_C(1) << 50);
-  const int64_t nq0 = k >= 1 ? tr[k - 1].nq_size : 1;
-  const double gq = k >= 2 ? grow(tr[k - 1].nq_size, tr[k - 2].nq_size, 1.0) : 16.0;
-  *nq = std::max<int64_t>(1, (int64_t)std::min((double)nq0 * gq, cap));
-  *fe = 0;
+  const double left = (double)std::max<int64_t>(0, hl.n_c - hl.visited - hl.unreach);
   if (mode_is_td(mode)) {
-    const int64_t fe0 = k >= 1 ? tr[k - 1].nq_edges : 0;
-    const double ge = k >= 2 ? grow(tr[k - 1].nq_edges, tr[k - 2].nq_edges, 32.0) : 32.0;
-    *fe = (int64_t)std::min((double)fe0 * std::max(ge, 1.0), cap);
+    const double fe0 = k >= 1 ? (double)tr[k - 1].nq_edges : 0.0;
+    *nq = (int64_t)std::max(1.0, std::min(std::max(fe0, 1.0), std::max(left, 1.0)));
+    *fe = (int64_t)std::min((double)hl.m, 24.0 * std::sqrt((double)hl.m * std::max(fe0, 1.0)));
+    return;
   }
+  const double nq0 = k >= 1 ? (double)tr[k - 1].nq_size : 1.0;
+  const double g = (k >= 2 && tr[k - 2].nq_size > 0) ? nq0 / (double)tr[k - 2].nq_size : 1.0;
+  *nq = (int64_t)std::max(1.0, std::min(nq0 * g, 0.8 * left));
+  *fe = 0;
 }
 
 // The step the host queues behind `last` before `last` has reported: the
@@ -1726,7 +1733,7 @@ uint32_t predict_next(const Step& last, HostLoop hl, const std::vector<g500_leve
     return step_code(step_level(last.code),
                      td_step_mode(tr.empty() ? 0 : tr.back().nq_size, 0, hl.exec_thr));
   int64_t nq, fe, at[G500_AT_NUM];
-  estimate_level(mode, tr, &nq, &fe);
+  estimate_level(mode, tr, hl, &nq, &fe);
   return hl.after_level(last.code, nq, fe, at);
 }
 
@@ -1743,7 +1750,7 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
     if (mode == kModeConv) continue;
     g500_level_trace t{};
     int64_t at[G500_AT_NUM];
-    estimate_level(mode, tr, &t.nq_size, &t.nq_edges);
+    estimate_level(mode, tr, h, &t.nq_size, &t.nq_edges);
     t.level = step_level(pend[i].code);
     t.top_down = mode_is_td(mode) ? 1 : 0;
     (void)h.after_level(pend[i].code, t.nq_size, t.nq_edges, at);
@@ -1757,7 +1764,7 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
     if (lm != kModeConv) {
       g500_level_trace t{};
       int64_t at[G500_AT_NUM];
-      estimate_level(lm, tr, &t.nq_size, &t.nq_edges);
+      estimate_level(lm, tr, h, &t.nq_size, &t.nq_edges);
       t.level = step_level(last.code);
       t.top_down = mode_is_td(lm) ? 1 : 0;
       (void)h.after_level(last.code, t.nq_size, t.nq_edges, at);
@@ -1852,6 +1859,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   hl.beta_thr = (int64_t)((double)n_c / (beta * edgefactor * 2.0));
   hl.exec_thr = b->exec_alpha > 0 ? (int64_t)std::floor((double)g->m / b->exec_alpha)
                                   : INT64_MAX;
+  hl.visited = root_in ? 1 : 0;
+  hl.unreach = b->unreach;
   const uint32_t first = step_code(root_level + 1, kModeTDS);
   LoopInit li{first, root_in ? 1 : 0, 0, hl.alpha_thr, hl.beta_thr, hl.exec_thr};
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
@@ -2010,6 +2019,9 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->stats.gpu_ms = ms;
   b->stats.nlevels = (int)b->trace.size();
   b->stats.mispredicted = mispredicted;
+  // vertices with edges this run left unreached: the next run's guesses
+  // take them to lie in other components
+  b->unreach = std::max<int64_t>(0, n_c - hl.visited);
   G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_g0, b->ev_g1));
   b->stats.gather_ms = ms;
   for (const Step& st : ran) {

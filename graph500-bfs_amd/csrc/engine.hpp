@@ -82,6 +82,7 @@ struct g500_bfs {
   // guessed steps queued behind the running one (G500_SPECULATE=<n>; 0: the
   // host waits for every step's report before it launches the next)
   int spec_depth = 1;
+  int64_t unreach = 0;  // vertices with edges the last run left unreached
 
   // validation scratch
   g500::DevBuf<unsigned long long> vcount;  // [G500_VERR_NUM + 2]
