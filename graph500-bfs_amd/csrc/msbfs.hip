@@ -194,6 +194,22 @@ struct MsLevel {
 };
 
 enum { kModeTopDown = 0, kModeStream = 1, kModePull = 2, kModeSplit = 3 };
+// rows of per-lane degree sums for a split level: block b of a top-down
+// level's count pass adds its sums to row b % kLaneParts
+constexpr int kLaneParts = 512;
+
+// Adds degree d to the LDS counter of every lane set in x.  A word with more
+// lanes set than clear adds d to the all-lanes counter and subtracts it from
+// its clear lanes (modulo 2^64), so no word costs more than 33 LDS atomics.
+__device__ inline void lane_add(unsigned long long* lfe, unsigned long long* lall, uint64_t x,
+                                unsigned long long d) {
+  if (__popcll(x) > 32) {
+    atomicAdd(lall, d);
+    for (uint64_t b = ~x; b; b &= b - 1) atomicAdd(&lfe[__ffsll((long long)b) - 1], 0ull - d);
+  } else {
+    for (uint64_t b = x; b; b &= b - 1) atomicAdd(&lfe[__ffsll((long long)b) - 1], d);
+  }
+}
 enum { kDenseStream = 1, kDenseHybrid = 2 };
 
 __device__ inline bool ms_active(const MsLevel& a) { return a.cnt[a.L] != 0; }
@@ -239,21 +255,24 @@ __device__ inline void ms_note_lanes(const MsLevel& a, uint64_t (&lanes)[K]) {
   }
 }
 
-// Level preparation: clear the next bitmap; for a top-down or streaming level
-// clear F_{L+1} (it is filled by atomicOr); when F_L came from a top-down
-// level and this level pushes too, fold F_L into seen (a level that pulls
-// ORs F_L into each vertex's seen word itself, and the other modes write
-// seen themselves).
+// Level preparation: clear the next bitmap; unless the level only pulls,
+// clear F_{L+1} (pushes fill it by atomicOr; a pulling level stores every
+// vertex's word); when F_L came from a top-down level and this level pushes
+// too, fold F_L into seen (a level that pulls ORs F_L into each vertex's
+// seen word itself, and the other modes write seen themselves); before a
+// top-down level, clear the per-lane degree sums its count pass adds up for
+// the next level.
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_prep(MsLevel a, int64_t bm_words) {
   if (!ms_active(a)) return;
-  // F_{L+1} filled by atomicOr (the pulls store every vertex's word)
   const int mode = ms_mode(a, a.L);
-  const bool clear = mode != kModePull && mode != kModeSplit;
+  const bool clear = mode != kModePull;
   const bool fold = a.L > 0 && ms_top_down(a, a.L - 1) && mode == kModeTopDown;
   const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * kT;
   for (int64_t i = i0; i < bm_words; i += stride) a.fbn[i] = 0;
+  if (mode == kModeTopDown && a.split_thr > 0)
+    for (int64_t i = i0; i < (int64_t)kLaneParts * 256; i += stride) a.lane_part[i] = 0;
   if (clear)
     for (int64_t i = i0; i < a.n_c * K; i += stride) a.fn[i] = 0;
   if (fold)
@@ -329,10 +348,20 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
 // After a top-down level's pushes: new (vertex, root) pairs = the lanes of
 // F_{L+1} (pushes left out lanes already seen), frontier degree sum = the
 // degrees of the vertices in its bitmap.  One thread per vertex.
+// With split levels on, it also adds up each lane's degree sum over F_{L+1}
+// (the next level, if it is a split level, picks its pushing lanes from
+// them): LDS counters per block, added to the block's lane_part row.
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_td_count(MsLevel a) {
   __shared__ uint64_t red[kWavesPerBlock];
+  __shared__ unsigned long long lfe[64 * K], lall[K];
   if (!ms_active(a) || !ms_top_down(a, a.L)) return;
+  const bool split = a.split_thr > 0;
+  if (split) {
+    for (int i = threadIdx.x; i < 64 * K; i += kT) lfe[i] = 0;
+    if (threadIdx.x < K) lall[threadIdx.x] = 0;
+    __syncthreads();
+  }
   uint64_t disc = 0, edges = 0, lanes[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) lanes[k] = 0;
@@ -341,12 +370,22 @@ __global__ __launch_bounds__(kT) void k_ms_td_count(MsLevel a) {
     if (!in_bitmap(a.fbn, v)) continue;
     uint64_t x[K];
     load_lanes<K>(a.fn + v * K, x);
+    const unsigned long long d = a.deg[v];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       disc += (uint64_t)__popcll(x[k]);
       lanes[k] |= x[k];
+      if (split) lane_add(lfe + 64 * k, lall + k, x[k], d);
     }
-    edges += a.deg[v];
+    edges += d;
+  }
+  if (split) {
+    __syncthreads();
+    unsigned long long* row = a.lane_part + (int64_t)(blockIdx.x % kLaneParts) * 256;
+    for (int i = threadIdx.x; i < 64 * K; i += kT) {
+      const unsigned long long x = lfe[i] + lall[i / 64];
+      if (x) atomicAdd(&row[i], x);
+    }
   }
   ms_note_lanes<K>(a, lanes);
   ms_flush(a, red, disc, edges);
@@ -664,43 +703,8 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push_heavy(MsLevel a,
 // rows of their frontier vertices, and the pulls stop once the other lanes
 // are covered.
 //
-// Per-lane degree sums of F_L: each block adds the degrees of its frontier
-// vertices into LDS counters, one per lane, and stores them to its own
-// lane_part row (kLaneParts rows of 256).
-constexpr int kLaneParts = 512;
-// A word with more lanes set than clear adds its degree to the word's
-// all-lanes counter and subtracts it from its clear lanes (modulo 2^64), so
-// no word costs more than 33 LDS atomics.
-template <int K>
-__global__ __launch_bounds__(kT) void k_ms_lane_fe(MsLevel a) {
-  __shared__ unsigned long long lfe[64 * K], lall[K];
-  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
-  for (int i = threadIdx.x; i < 64 * K; i += kT) lfe[i] = 0;
-  if (threadIdx.x < K) lall[threadIdx.x] = 0;
-  __syncthreads();
-  for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < a.n_c;
-       v += (int64_t)gridDim.x * kT) {
-    if (!in_bitmap(a.fb, v)) continue;
-    uint64_t x[K];
-    load_lanes<K>(a.fl + v * K, x);
-    const unsigned long long d = a.deg[v];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if (__popcll(x[k]) > 32) {
-        atomicAdd(&lall[k], d);
-        for (uint64_t b = ~x[k]; b; b &= b - 1)
-          atomicAdd(&lfe[64 * k + __ffsll((long long)b) - 1], 0ull - d);
-      } else {
-        for (uint64_t b = x[k]; b; b &= b - 1)
-          atomicAdd(&lfe[64 * k + __ffsll((long long)b) - 1], d);
-      }
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 64 * K; i += kT)
-    a.lane_part[(int64_t)blockIdx.x * 256 + i] = lfe[i] + lall[i / 64];
-}
-
+// Per-lane degree sums of F_L come from the count pass of the top-down level
+// that produced it (k_ms_td_count).
 // One block: sums the lane_part rows per lane (each wave a quarter of the
 // rows, 16 loads in flight per lane) and sets split_mask to the lanes
 // present in F_L whose frontier degree sum is below split_thr.
@@ -741,9 +745,10 @@ __global__ __launch_bounds__(kT) void k_ms_lane_mask(MsLevel a, int parts) {
   if (t < 4) a.split_mask[t] = mk[t];
 }
 
-// Push side of a split level: the frontier vertices of [vb, ve), G lanes
-// each, OR their pushing lanes into F_{L+1} along their rows (k_ms_commit
-// then masks with seen).
+// Push side of a split level, queued before the pulls: the frontier
+// vertices of [vb, ve), G lanes each, OR their pushing lanes into F_{L+1}
+// (cleared by k_ms_prep) along their rows; each vertex's pull then adds its
+// own lanes and masks the whole word with seen.
 template <int K, int G>
 __global__ __launch_bounds__(kT) void k_ms_split_push(MsLevel a, int64_t vb, int64_t ve) {
   if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
@@ -858,8 +863,8 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
   if (!ms_active(a)) return;
   const int mode = ms_mode(a, a.L);
   if (mode != kModePull && mode != kModeSplit) return;
-  // split level: the pushing lanes count as covered too, and the raw lanes
-  // found go to F_{L+1} for k_ms_commit (which masks, counts and marks)
+  // split level: the pushing lanes count as covered too, and the lanes the
+  // pushes (queued first) put into F_{L+1} join the lanes pulled
   const bool split = mode == kModeSplit;
   const int lig = threadIdx.x % G;
   const int64_t gid0 = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
@@ -892,6 +897,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
       uint64_t s0[K], f[K], s[K];
       load_lanes<K>(a.seen + v * K, s0);
       load_lanes<K>(a.fl + v * K, f);
+      if (split) load_lanes<K>(a.fn + v * K, acc);
       bool full = true;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -927,18 +933,13 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
           for (int k = 0; k < K; ++k) cov = cov && ((acc[k] | s[k] | inact[k]) == ~0ull);
           if (cov) break;
         }
-        if (!split) {
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            nxt[k] = acc[k] & ~s[k];
-            nz = nz || nxt[k] != 0;
-          }
-        }
       }
-      if (lig == 0 && split) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) a.fn[v * K + k] = acc[k];
-      } else if (lig == 0) {
+      for (int k = 0; k < K; ++k) {
+        nxt[k] = acc[k] & ~s[k];
+        nz = nz || nxt[k] != 0;
+      }
+      if (lig == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           a.fn[v * K + k] = nxt[k];
@@ -1050,21 +1051,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy_fin(MsLevel a, int64_t h,
   __shared__ uint64_t red[kWavesPerBlock];
   if (!ms_active(a)) return;
   const int mode = ms_mode(a, a.L);
-  if (mode == kModeSplit) {
-    // raw lanes to F_{L+1}; k_ms_commit masks, counts and marks them
-    for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < h;
-         v += (int64_t)gridDim.x * kT) {
-      uint64_t x[K];
-      load_lanes<K>(gacc + v * K, x);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        a.fn[v * K + k] = x[k];
-        if (x[k]) gacc[v * K + k] = 0;
-      }
-    }
-    return;
-  }
-  if (mode != kModePull) return;
+  if (mode != kModePull && mode != kModeSplit) return;
   uint64_t disc = 0, edges = 0, lanes[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) lanes[k] = 0;
@@ -1074,6 +1061,12 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy_fin(MsLevel a, int64_t h,
     load_lanes<K>(a.seen + v * K, s0);
     load_lanes<K>(a.fl + v * K, f);
     load_lanes<K>(gacc + v * K, x);
+    if (mode == kModeSplit) {  // the lanes the pushes put there join
+      uint64_t p[K];
+      load_lanes<K>(a.fn + v * K, p);
+#pragma unroll
+      for (int k = 0; k < K; ++k) x[k] |= p[k];
+    }
     bool nz = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1518,11 +1511,23 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
                        (int)kModeStream);
     G500_CHECK_LAUNCH();
   }
-  // split level: per-lane frontier degree sums and the pushing lanes first
-  hipLaunchKernelGGL((k_ms_lane_fe<K>), dim3(kLaneParts), dim3(kT), 0, s, a);
-  G500_CHECK_LAUNCH();
+  // split level: the pushing lanes, then their pushes, before the pulls
+  // (which take the pushed lanes into F_{L+1} with their own)
   hipLaunchKernelGGL((k_ms_lane_mask<K>), dim3(1), dim3(kT), 0, s, a, kLaneParts);
   G500_CHECK_LAUNCH();
+  if (b->n_items > 0) {
+    hipLaunchKernelGGL((k_ms_split_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
+                       b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
+    G500_CHECK_LAUNCH();
+  }
+  switch (b->tuning.medium_group) {
+    case 2: launch_split_push<K, 2>(b, a, h, m, s); break;
+    case 4: launch_split_push<K, 4>(b, a, h, m, s); break;
+    case 8: launch_split_push<K, 8>(b, a, h, m, s); break;
+    case 16: launch_split_push<K, 16>(b, a, h, m, s); break;
+    default: launch_split_push<K, 32>(b, a, h, m, s); break;
+  }
+  launch_split_push<K, 1>(b, a, m, n_c, s);
   if (b->n_items > 0) {
     hipLaunchKernelGGL((k_ms_bu_heavy<K, 4>), dim3(hgrid), dim3(kT), 0, s, a,
                        b->item_start.get(), b->item_vertex.get(), b->n_items, chunk,
@@ -1540,24 +1545,6 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
     default: launch_bu<K, 32, 2>(b, a, h, m, s); break;
   }
   launch_bu<K, 1, 4>(b, a, m, n_c, s);
-  // split level: the light roots push after the pulls stored F_{L+1}, then
-  // the commit masks, counts and marks
-  if (b->n_items > 0) {
-    hipLaunchKernelGGL((k_ms_split_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
-                       b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
-    G500_CHECK_LAUNCH();
-  }
-  switch (b->tuning.medium_group) {
-    case 2: launch_split_push<K, 2>(b, a, h, m, s); break;
-    case 4: launch_split_push<K, 4>(b, a, h, m, s); break;
-    case 8: launch_split_push<K, 8>(b, a, h, m, s); break;
-    case 16: launch_split_push<K, 16>(b, a, h, m, s); break;
-    default: launch_split_push<K, 32>(b, a, h, m, s); break;
-  }
-  launch_split_push<K, 1>(b, a, m, n_c, s);
-  hipLaunchKernelGGL((k_ms_commit<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a,
-                     (int)kModeSplit);
-  G500_CHECK_LAUNCH();
 }
 
 void enqueue_level(const g500_msbfs* b, int K, int L, hipStream_t s) {
