@@ -54,6 +54,8 @@
 // synchronisations per level.
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <deque>
 #include <cstdlib>
@@ -274,6 +276,7 @@ __global__ __launch_bounds__(kT) void k_bfs_init(uint64_t* __restrict__ vis,
     ls->alpha_thr = li.alpha_thr;
     ls->beta_thr = li.beta_thr;
     ls->exec_thr = li.exec_thr;
+    ls->snap_tag = 0;
     if (root_new >= 0 && root_new < n_c) {
       pred_new[root_new] = entry;
       q[0] = (uint32_t)root_new;
@@ -1098,11 +1101,29 @@ __device__ inline void queue_wave_span(int64_t words, int64_t* a, int64_t* e) {
   *e = min(*a + per_wave, b1);
 }
 
+// Gates between the engine stream and the gather stream: a value in signal
+// memory that a wait packet on the other stream compares; released to system
+// scope so the packet processor reads it from memory.
+__device__ inline void open_gate(unsigned long long* gate, unsigned long long no) {
+  __hip_atomic_store(gate, no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__global__ void k_open_gate(unsigned long long* gate, unsigned long long no) {
+  if (threadIdx.x == 0) open_gate(gate, no);
+}
+
 __global__ __launch_bounds__(kT) void k_queue_count(const uint64_t* __restrict__ cur,
                                                    const uint64_t* __restrict__ prev,
                                                    int64_t words, uint32_t* __restrict__ wcnt,
                                                    uint64_t* __restrict__ snap,
-                                                   const LevelSync* ls, uint32_t code) {
+                                                   LevelSync* ls, uint32_t code,
+                                                   unsigned long long* gate,
+                                                   unsigned long long gate_no) {
+  // a conversion opens the gather's gate whether it runs or not (the levels
+  // before it have ended; the gather checks snap_tag)
+  if (gate && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (step_is_next(ls, code)) ls->snap_tag = code;
+    open_gate(gate, gate_no);
+  }
   if (!step_is_next(ls, code)) return;
   int64_t a, e;
   queue_wave_span(words, &a, &e);
@@ -1243,13 +1264,21 @@ __device__ inline int64_t run_entry(int64_t e, uint32_t epoch) {
                                                      : (int64_t)-1;
 }
 
+// A gather queued on a guess that a conversion runs (snap_code != 0) returns
+// at once unless that conversion wrote the bitmap snapshot.
+__device__ inline bool snapshot_is(const LevelSync* ls, uint32_t snap_code) {
+  return snap_code == 0 || __builtin_amdgcn_readfirstlane(ls->snap_tag) == snap_code;
+}
+
 __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__ o2n,
                                                    uint32_t epoch,
                                                    const int64_t* __restrict__ pred_new,
                                                    int64_t n, int64_t n_c, int64_t root,
                                                    const uint32_t* __restrict__ tree_parent,
                                                    const uint64_t* __restrict__ has_parent,
-                                                   int64_t* __restrict__ pred) {
+                                                   int64_t* __restrict__ pred,
+                                                   const LevelSync* ls, uint32_t snap_code) {
+  if (!snapshot_is(ls, snap_code)) return;
   const XcdSpan xs = xcd_span(n);
   for (int64_t o = xs.lo + xs.rank * kT + threadIdx.x; o < xs.hi; o += xs.nblk * kT) {
     const uint32_t v = o2n[o];
@@ -1272,7 +1301,9 @@ __global__ __launch_bounds__(kT) void k_pred_gather_core(const uint2* __restrict
                                                         int64_t n_core, uint32_t epoch,
                                                         const int64_t* __restrict__ pred_new,
                                                         int64_t root,
-                                                        int64_t* __restrict__ pred) {
+                                                        int64_t* __restrict__ pred,
+                                                        const LevelSync* ls, uint32_t snap_code) {
+  if (!snapshot_is(ls, snap_code)) return;
   const XcdSpan xs = xcd_span(n_core);
   for (int64_t i = xs.lo + xs.rank * kT + threadIdx.x; i < xs.hi; i += xs.nblk * kT) {
     const uint2 ov = get_pair(core + i);
@@ -1298,13 +1329,16 @@ __global__ __launch_bounds__(kT) void k_core_list(const uint32_t* __restrict__ f
 
 // pred entries of the vertices reached after the snapshot the early gather
 // used (cur & ~snap): one thread per bitmap word, entries scattered by
-// original ID.
+// original ID.  Queued on a guess that the run ends (done_code != 0), it
+// returns at once unless the loop state says so.
 __global__ __launch_bounds__(kT) void k_pred_fixup(const uint64_t* __restrict__ cur,
                                                   const uint64_t* __restrict__ snap,
                                                   int64_t words,
                                                   const uint32_t* __restrict__ n2o,
                                                   const int64_t* __restrict__ pred_new,
-                                                  int64_t* __restrict__ pred) {
+                                                  int64_t* __restrict__ pred,
+                                                  const LevelSync* ls, uint32_t done_code) {
+  if (done_code != 0 && !step_is_next(ls, done_code)) return;
   const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (w >= words) return;
   uint64_t bits = cur[w] & ~snap[w];
@@ -1341,6 +1375,43 @@ unsigned grid_for(int64_t work_items, int64_t per_block) {
 }
 
 }  // namespace
+
+// Gates live in signal memory, which is not freed once a wait packet has
+// read it: engines return theirs to a per-device pool for the next engine.
+namespace {
+std::mutex g_gate_mu;
+std::map<int, std::vector<unsigned long long*>> g_gate_pool;
+}  // namespace
+
+void take_gates(g500_bfs* b) {
+  const int dev = b->g->device;
+  unsigned long long* slot[2] = {nullptr, nullptr};
+  {
+    std::lock_guard<std::mutex> lk(g_gate_mu);
+    auto& pool = g_gate_pool[dev];
+    for (auto& p : slot)
+      if (!pool.empty()) {
+        p = pool.back();
+        pool.pop_back();
+      }
+  }
+  for (auto& p : slot) {
+    if (!p) G500_CHECK_HIP(hipExtMallocWithFlags((void**)&p, sizeof(unsigned long long),
+                                                 hipMallocSignalMemory));
+    G500_CHECK_HIP(hipMemset(p, 0, sizeof(unsigned long long)));
+  }
+  b->gate_conv = slot[0];
+  b->gate_gather = slot[1];
+  b->gate_device = dev;
+  b->gathers = 0;
+}
+
+void return_gates(g500_bfs* b) {
+  std::lock_guard<std::mutex> lk(g_gate_mu);
+  for (unsigned long long* p : {b->gate_conv, b->gate_gather})
+    if (p) g_gate_pool[b->gate_device].push_back(p);
+  b->gate_conv = b->gate_gather = nullptr;
+}
 
 void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   g500_graph* g = b->g;
@@ -1389,8 +1460,7 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
-  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_snap, hipEventDisableTiming));
-  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
+  take_gates(b);
   G500_CHECK_HIP(hipEventCreate(&b->ev_g0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_g1));
   G500_CHECK_HIP(hipStreamCreate(&b->gstream));
@@ -1476,8 +1546,9 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
 
 // pred by original ID from this run's level entries (a smaller grid beside
 // the last top-down levels measured slower: the random reads need every
-// wave the chip holds).
-void launch_gather(g500_bfs* b, int64_t root, hipStream_t st) {
+// wave the chip holds).  snap_code != 0: queued before the conversion of
+// that code is known to run (the kernel checks).
+void launch_gather(g500_bfs* b, int64_t root, hipStream_t st, uint32_t snap_code) {
   g500_graph* g = b->g;
   G500_CHECK_HIP(hipEventRecord(b->ev_g0, st));
   // grids are a multiple of kXcds (xcd_span)
@@ -1487,12 +1558,13 @@ void launch_gather(g500_bfs* b, int64_t root, hipStream_t st) {
   if (b->n_core > 0)
     hipLaunchKernelGGL(k_pred_gather_core, dim3(grid(b->n_core)), dim3(kT), 0, st,
                        b->core.get(), b->n_core, b->epoch, b->pred_new.get(), root,
-                       b->pred.get());
+                       b->pred.get(), b->lsync.get(), snap_code);
   else
     hipLaunchKernelGGL(k_pred_gather, dim3(grid(g->n)),
                        dim3(kT), 0, st, g->orig2new.get(), b->epoch, b->pred_new.get(), g->n,
                        g->n_c, root, g->corebfs ? g->tree_parent.get() : nullptr,
-                       g->corebfs ? g->has_parent.get() : nullptr, b->pred.get());
+                       g->corebfs ? g->has_parent.get() : nullptr, b->pred.get(),
+                       b->lsync.get(), snap_code);
   G500_CHECK_LAUNCH();
   G500_CHECK_HIP(hipEventRecord(b->ev_g1, st));
 }
@@ -1532,7 +1604,7 @@ hipEvent_t pool_event(std::vector<hipEvent_t>& pool, size_t i) {
 // no bottom-up level uses (a top-down level run as a sweep refills it from
 // the bitmap afterwards); a top-down level swaps the queues; the conversion
 // at the switch to top-down writes the queue and the gather's bitmap
-// snapshot (ev_snap is recorded as soon as that is written).
+// snapshot and opens the gather's gate with its report number.
 Step launch_step(g500_bfs* b, uint32_t code, const Bufs& in, size_t* nlev_ev,
                  size_t* nsweep_ev) {
   g500_graph* g = b->g;
@@ -1577,9 +1649,9 @@ Step launch_step(g500_bfs* b, uint32_t code, const Bufs& in, size_t* nlev_ev,
                       uint64_t* snap) {
     const dim3 grid((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(b->words, (int64_t)kT)));
     hipLaunchKernelGGL(k_queue_count, grid, dim3(kT), 0, s, cur, prev, b->words, b->qcnt.get(),
-                       snap, ls, code);
+                       snap, ls, code, snap ? b->gate_conv : nullptr,
+                       (unsigned long long)st.seq);
     G500_CHECK_LAUNCH();
-    if (snap) G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
     hipLaunchKernelGGL(k_queue_write, grid, dim3(kT), 0, s, cur, prev, b->words,
                        g->deg_new.get(), b->qcnt.get(), out, b->qedges.get(), ls, code);
     G500_CHECK_LAUNCH();
@@ -1699,7 +1771,8 @@ struct HostLoop {
 // Frontier totals (size, degree sum) that a level of mode `mode` is expected
 // to find, from the last level of `tr` (reported or itself estimated).  A
 // top-down level over a frontier of degree sum fe0 finds about fe0 new
-// vertices (early on, most edges lead to unvisited ones), whose degree sum
+// vertices (early on, most edges lead to unvisited ones; none once every
+// vertex outside the other components is reached), whose degree sum
 // on Kronecker graphs is about 24 sqrt(m fe0) (the frontier reaches the hubs
 // at once: fitted to SCALE-24 and 26 runs, within a factor of two from fe0
 // ~ 10^3 to 10^7).  A bottom-up level finds the last level's count times its
@@ -1711,7 +1784,7 @@ void estimate_level(uint32_t mode, const std::vector<g500_level_trace>& tr, cons
   const double left = (double)std::max<int64_t>(0, hl.n_c - hl.visited - hl.unreach);
   if (mode_is_td(mode)) {
     const double fe0 = k >= 1 ? (double)tr[k - 1].nq_edges : 0.0;
-    *nq = (int64_t)std::max(1.0, std::min(std::max(fe0, 1.0), std::max(left, 1.0)));
+    *nq = (int64_t)std::min(std::max(fe0, 1.0), left);
     *fe = (int64_t)std::min((double)hl.m, 24.0 * std::sqrt((double)hl.m * std::max(fe0, 1.0)));
     return;
   }
@@ -1737,11 +1810,54 @@ uint32_t predict_next(const Step& last, HostLoop hl, const std::vector<g500_leve
   return hl.after_level(last.code, nq, fe, at);
 }
 
+// The end of a run queued on guesses: the pred gather behind a guessed
+// conversion (k_pred_gather* checks that the conversion ran) and the fixup
+// behind the step guessed to be the last (k_pred_fixup checks that the loop
+// ended there).
+struct EndGuess {
+  int64_t root = 0;
+  bool gather_confirmed = false;     // a gather follows the conversion that ran
+  unsigned long long conv_seq = 0;   // the guessed conversion the last guessed gather follows
+  unsigned long long fixup_seq = 0;  // the step the guessed fixup follows
+  uint32_t fixup_code = 0;           // the final code it waits for
+};
+
+// The engine stream waits until every gather queued so far has ended.
+void wait_gathers(g500_bfs* b) {
+  if (b->gathers > 0)
+    G500_CHECK_HIP(hipStreamWaitValue64(b->stream, b->gate_gather, b->gathers,
+                                        hipStreamWaitValueGte, ~0ull));
+}
+
+// The gather on gstream, behind the conversion with report number conv_seq
+// (snap_code: its code when it is queued on a guess), then the gate the
+// engine stream waits on.
+void queue_gather(g500_bfs* b, int64_t root, unsigned long long conv_seq, uint32_t snap_code) {
+  G500_CHECK_HIP(hipStreamWaitValue64(b->gstream, b->gate_conv, conv_seq, hipStreamWaitValueGte,
+                                      ~0ull));
+  launch_gather(b, root, b->gstream, snap_code);
+  hipLaunchKernelGGL(k_open_gate, dim3(1), dim3(kWave), 0, b->gstream, b->gate_gather,
+                     ++b->gathers);
+  G500_CHECK_LAUNCH();
+}
+
+void queue_fixup(g500_bfs* b, const uint64_t* cur, uint32_t done_code) {
+  hipStream_t s = b->stream;
+  wait_gathers(b);
+  hipLaunchKernelGGL(k_pred_fixup, dim3((unsigned)ceil_div(b->words, kT)), dim3(kT), 0, s, cur,
+                     b->vis_snap.get(), b->words, b->g->new2orig.get(), b->pred_new.get(),
+                     b->pred.get(), b->lsync.get(), done_code);
+  G500_CHECK_LAUNCH();
+}
+
 // Queues guessed steps behind the queued ones until `depth` are queued:
 // the steps queued after the first one (whose predecessor has reported)
-// contribute their expected totals to the guess of the next one.
+// contribute their expected totals to the guess of the next one.  A guessed
+// conversion gets the pred gather behind it on the second stream; a step
+// guessed to end the run gets the fixup behind it once the gather is
+// confirmed.
 void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t depth,
-               size_t* nlev_ev, size_t* nsweep_ev) {
+               size_t* nlev_ev, size_t* nsweep_ev, EndGuess* eg) {
   if (pend.empty() || pend.size() >= depth) return;
   HostLoop h = hl;
   std::vector<g500_level_trace> tr = b->trace;
@@ -1759,7 +1875,14 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
   while (pend.size() < depth) {
     const Step& last = pend.back();
     const uint32_t next = predict_next(last, h, tr);
-    if (step_mode(next) == kModeDone) return;
+    if (step_mode(next) == kModeDone) {
+      if (eg->gather_confirmed && eg->fixup_seq != last.seq) {
+        queue_fixup(b, last.after.vis, next);
+        eg->fixup_seq = last.seq;
+        eg->fixup_code = next;
+      }
+      return;
+    }
     const uint32_t lm = step_mode(last.code);
     if (lm != kModeConv) {
       g500_level_trace t{};
@@ -1771,6 +1894,10 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
       tr.push_back(t);
     }
     pend.push_back(launch_step(b, next, last.after, nlev_ev, nsweep_ev));
+    if (step_mode(next) == kModeConv && !eg->gather_confirmed) {
+      queue_gather(b, eg->root, pend.back().seq, next);
+      eg->conv_seq = pend.back().seq;
+    }
   }
 }
 
@@ -1822,12 +1949,12 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   // this run's epoch; before the epochs wrap, every entry is set to the
   // epoch no run uses, so no entry of an earlier run can match
   if (++b->epoch >= kEpochNone) {
-    G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
+    wait_gathers(b);
     G500_CHECK_HIP(hipMemsetAsync(b->pred_new.get(), 0xFF, b->pred_new.n * sizeof(int64_t), s));
     b->epoch = 0;
   }
   // a gather left running by a run that failed must not overlap this one
-  G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
+  wait_gathers(b);
   if (b->n_core > 0 && b->pred_fill_needed) {
     // entries of vertices without edges are -1 and stay so; the fill is
     // redone only after the buffer was overwritten (validation of a host pred)
@@ -1876,6 +2003,9 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   int64_t qlen = root_in ? 1 : 0;
   int64_t est_edges = -1;  // degree sum of the frontier (unknown for the root)
   bool early_gather = false;  // pred gather already running on gstream
+  EndGuess eg;
+  eg.root = root;
+  uint32_t final_code = 0;
   int64_t last_nq = 0;
   unsigned long long rep[4];
   size_t nlev_ev = 0, nsweep_ev = 0;
@@ -1887,7 +2017,8 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
                              &nlev_ev, &nsweep_ev));
   Bufs cur{};
   for (;;) {
-    if (b->spec_depth > 0) speculate(b, pend, hl, (size_t)b->spec_depth + 1, &nlev_ev, &nsweep_ev);
+    if (b->spec_depth > 0)
+      speculate(b, pend, hl, (size_t)b->spec_depth + 1, &nlev_ev, &nsweep_ev, &eg);
     const Step st = pend.front();
     pend.pop_front();
     wait_report(b, st.seq, rep);
@@ -1951,7 +2082,10 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       fail(G500_ERR_STATE, "level %d: the device chose step %u:%u, the reference rule %u:%u",
            step_level(st.code), step_level((uint32_t)rep[2]), step_mode((uint32_t)rep[2]),
            step_level(next), step_mode(next));
-    if (step_mode(next) == kModeDone) break;
+    if (step_mode(next) == kModeDone) {
+      final_code = next;
+      break;
+    }
     if (pend.empty() || pend.front().code != next) {
       // predicted wrong: the queued steps return at once
       mispredicted += (int)pend.size();
@@ -1961,14 +2095,13 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     if (step_mode(next) == kModeConv) {
       // switching to top-down: the remaining top-down levels are short and
       // latency-bound, so the pred gather of every vertex reached so far
-      // starts on the second stream once the conversion has written the
-      // bitmap snapshot (ev_snap, recorded when the conversion was queued);
-      // the vertices the last levels reach are patched in at the end
-      // (k_pred_fixup)
-      G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
-      launch_gather(b, root, b->gstream);
-      G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
+      // starts on the second stream as soon as the conversion starts (its
+      // gate); the vertices the last levels reach are patched in at the end
+      // (k_pred_fixup).  A conversion queued on a guess has its gather
+      // queued behind it already.
+      if (eg.conv_seq != pend.front().seq) queue_gather(b, root, pend.front().seq, 0);
       early_gather = true;
+      eg.gather_confirmed = true;
     }
   }
   // the steps still queued (guesses past the last level) return at once
@@ -1977,15 +2110,13 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->q.p = cur.q;
   b->nq.p = cur.nq;
   if (early_gather) {
-    G500_CHECK_HIP(hipStreamWaitEvent(s, b->ev_gather, 0));
-    hipLaunchKernelGGL(k_pred_fixup, dim3((unsigned)ceil_div(words, kT)), dim3(kT), 0, s,
-                       b->vis.get(), b->vis_snap.get(), words, g->new2orig.get(),
-                       b->pred_new.get(), b->pred.get());
-    G500_CHECK_LAUNCH();
+    // unless the guessed fixup followed the step that ended the run
+    if (!(eg.fixup_seq == ran.back().seq && eg.fixup_code == final_code))
+      queue_fixup(b, b->vis.get(), 0);
     // both bitmaps; per vertex its new -> original ID, entry read and written
     alg += 16 * words + 20 * after_snap;
   } else {
-    launch_gather(b, root, s);
+    launch_gather(b, root, s, 0);
   }
   // the gather: per vertex with edges its original ID, new ID, entry read and
   // written; with CoreBFS every original ID (new ID and entry written) and
@@ -2089,12 +2220,11 @@ g500_bfs::~g500_bfs() {
   if (h_report) (void)hipHostFree(h_report);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
-  if (ev_snap) (void)hipEventDestroy(ev_snap);
-  if (ev_gather) (void)hipEventDestroy(ev_gather);
   if (ev_g0) (void)hipEventDestroy(ev_g0);
   if (ev_g1) (void)hipEventDestroy(ev_g1);
   for (auto e : sweep_events) (void)hipEventDestroy(e);
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto e : level_events) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
+  g500::return_gates(this);
 }

@@ -44,6 +44,7 @@ struct LevelSync {
   long long qedges;    // its degree sum (>= its CSR entries)
   long long prev_nq;   // frontier size of the previous level
   long long alpha_thr, beta_thr, exec_thr;  // int64(m/alpha), int64(n_c/(beta*ef*2)), m/exec_alpha
+  uint32_t snap_tag;   // code of the conversion that wrote the gather's bitmap snapshot (0: none)
 };
 
 // Host-visible (pinned, coherent, mapped) step reports: a ring of slots, the
@@ -90,11 +91,20 @@ struct g500_bfs {
 
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   // the pred gather of the vertices reached by the last bottom-up level runs
-  // on gstream during the remaining top-down levels, against vis_snap (the
-  // bitmap copy the conversion's count pass writes)
+  // on gstream during the remaining top-down levels; vis_snap is the bitmap
+  // it covers (copied by the conversion's count pass), the fixup adds the rest
   hipStream_t gstream = nullptr;
-  hipEvent_t ev_snap = nullptr, ev_gather = nullptr;
   g500::DevBuf<uint64_t> vis_snap;
+  // stream gates in signal memory (values only grow; never freed, as memory
+  // a wait packet has read): gate_conv = report number of the last
+  // conversion whose count pass has started (gstream waits on it before a
+  // gather), gate_gather = number of the last gather done (the engine stream
+  // waits on it before the fixup and the next run).  A wait packet on a
+  // value wakes about 1.5 us after the write, an event wait about 11 us.
+  unsigned long long* gate_conv = nullptr;
+  unsigned long long* gate_gather = nullptr;
+  unsigned long long gathers = 0;  // gathers queued on gstream so far
+  int gate_device = -1;
   g500::DevBuf<uint64_t> lpart;   // per-block totals of a level's kernels
   g500::DevBuf<uint32_t> qcnt;    // per-wave counts of the bitmap -> queue conversion
   g500::DevBuf<uint64_t> qedges;  // its per-block degree sums
@@ -115,7 +125,7 @@ struct g500_bfs {
   // slot per block (sweep, long-row and list kernels in that order)
   bool counting = false;
   g500::DevBuf<uint64_t> wacc;
-  double exec_alpha = 15.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
+  double exec_alpha = 25.0;  // > 0: top-down levels with > m/exec_alpha edges run bottom-up
 
   bool have_result = false;
   int64_t last_root = -1;
@@ -134,4 +144,6 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
 void bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
                   int64_t* te, int64_t* errs, int* ok);
 void bfs_levels(g500_bfs* b, uint16_t* out);
+void take_gates(g500_bfs* b);    // from the device's pool, cleared
+void return_gates(g500_bfs* b);  // back to the pool
 }  // namespace g500

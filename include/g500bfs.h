@@ -254,7 +254,7 @@ int g500_bfs_set_timing(g500_bfs* b, int on);
  * the rule runs top-down is executed as a bottom-up sweep when its frontier
  * holds more than m / exec_alpha edges (same vertices, same depths, any valid
  * parents).  0 disables (every top-down level runs top-down).  The default
- * comes from G500_EXEC_ALPHA in the environment, else 15 (on MI355X a
+ * comes from G500_EXEC_ALPHA in the environment, else 25 (on MI355X a
  * bottom-up sweep beats the top-down expansion of such levels). */
 int g500_bfs_set_exec_alpha(g500_bfs* b, double exec_alpha);
 /* Counting runs (on != 0): the bottom-up kernels also tally the bytes their
