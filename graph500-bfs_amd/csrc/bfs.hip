@@ -1101,14 +1101,11 @@ __device__ inline void queue_wave_span(int64_t words, int64_t* a, int64_t* e) {
   *e = min(*a + per_wave, b1);
 }
 
-// Gates between the engine stream and the gather stream: a value in signal
-// memory that a wait packet on the other stream compares; released to system
-// scope so the packet processor reads it from memory.
+// The gather's gate: a value in signal memory that a value wait on the
+// gather stream compares; released to system scope so the wait reads it
+// from memory.
 __device__ inline void open_gate(unsigned long long* gate, unsigned long long no) {
   __hip_atomic_store(gate, no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-__global__ void k_open_gate(unsigned long long* gate, unsigned long long no) {
-  if (threadIdx.x == 0) open_gate(gate, no);
 }
 
 __global__ __launch_bounds__(kT) void k_queue_count(const uint64_t* __restrict__ cur,
@@ -1376,41 +1373,36 @@ unsigned grid_for(int64_t work_items, int64_t per_block) {
 
 }  // namespace
 
-// Gates live in signal memory, which is not freed once a wait packet has
-// read it: engines return theirs to a per-device pool for the next engine.
+// Gates live in signal memory, which is not freed once a wait has read it:
+// engines return theirs to a per-device pool for the next engine.
 namespace {
 std::mutex g_gate_mu;
 std::map<int, std::vector<unsigned long long*>> g_gate_pool;
 }  // namespace
 
-void take_gates(g500_bfs* b) {
+void take_gate(g500_bfs* b) {
   const int dev = b->g->device;
-  unsigned long long* slot[2] = {nullptr, nullptr};
+  unsigned long long* p = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_gate_mu);
     auto& pool = g_gate_pool[dev];
-    for (auto& p : slot)
-      if (!pool.empty()) {
-        p = pool.back();
-        pool.pop_back();
-      }
+    if (!pool.empty()) {
+      p = pool.back();
+      pool.pop_back();
+    }
   }
-  for (auto& p : slot) {
-    if (!p) G500_CHECK_HIP(hipExtMallocWithFlags((void**)&p, sizeof(unsigned long long),
-                                                 hipMallocSignalMemory));
-    G500_CHECK_HIP(hipMemset(p, 0, sizeof(unsigned long long)));
-  }
-  b->gate_conv = slot[0];
-  b->gate_gather = slot[1];
+  if (!p)
+    G500_CHECK_HIP(hipExtMallocWithFlags((void**)&p, sizeof(unsigned long long),
+                                         hipMallocSignalMemory));
+  G500_CHECK_HIP(hipMemset(p, 0, sizeof(unsigned long long)));
+  b->gate_conv = p;
   b->gate_device = dev;
-  b->gathers = 0;
 }
 
-void return_gates(g500_bfs* b) {
+void return_gate(g500_bfs* b) {
   std::lock_guard<std::mutex> lk(g_gate_mu);
-  for (unsigned long long* p : {b->gate_conv, b->gate_gather})
-    if (p) g_gate_pool[b->gate_device].push_back(p);
-  b->gate_conv = b->gate_gather = nullptr;
+  if (b->gate_conv) g_gate_pool[b->gate_device].push_back(b->gate_conv);
+  b->gate_conv = nullptr;
 }
 
 void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
@@ -1460,7 +1452,8 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
-  take_gates(b);
+  take_gate(b);
+  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
   G500_CHECK_HIP(hipEventCreate(&b->ev_g0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_g1));
   G500_CHECK_HIP(hipStreamCreate(&b->gstream));
@@ -1823,22 +1816,15 @@ struct EndGuess {
 };
 
 // The engine stream waits until every gather queued so far has ended.
-void wait_gathers(g500_bfs* b) {
-  if (b->gathers > 0)
-    G500_CHECK_HIP(hipStreamWaitValue64(b->stream, b->gate_gather, b->gathers,
-                                        hipStreamWaitValueGte, ~0ull));
-}
+void wait_gathers(g500_bfs* b) { G500_CHECK_HIP(hipStreamWaitEvent(b->stream, b->ev_gather, 0)); }
 
 // The gather on gstream, behind the conversion with report number conv_seq
-// (snap_code: its code when it is queued on a guess), then the gate the
-// engine stream waits on.
+// (snap_code: its code when it is queued on a guess); ev_gather follows it.
 void queue_gather(g500_bfs* b, int64_t root, unsigned long long conv_seq, uint32_t snap_code) {
   G500_CHECK_HIP(hipStreamWaitValue64(b->gstream, b->gate_conv, conv_seq, hipStreamWaitValueGte,
                                       ~0ull));
   launch_gather(b, root, b->gstream, snap_code);
-  hipLaunchKernelGGL(k_open_gate, dim3(1), dim3(kWave), 0, b->gstream, b->gate_gather,
-                     ++b->gathers);
-  G500_CHECK_LAUNCH();
+  G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
 }
 
 void queue_fixup(g500_bfs* b, const uint64_t* cur, uint32_t done_code) {
@@ -1894,7 +1880,10 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
       tr.push_back(t);
     }
     pend.push_back(launch_step(b, next, last.after, nlev_ev, nsweep_ev));
-    if (step_mode(next) == kModeConv && !eg->gather_confirmed) {
+    // (only behind a list level: the gate's wait polls until the conversion
+    // starts, and a list level before the switch is short)
+    if (step_mode(next) == kModeConv && !eg->gather_confirmed &&
+        step_mode(last.code) == kModeBUList) {
       queue_gather(b, eg->root, pend.back().seq, next);
       eg->conv_seq = pend.back().seq;
     }
@@ -2226,5 +2215,6 @@ g500_bfs::~g500_bfs() {
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto e : level_events) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
-  g500::return_gates(this);
+  if (ev_gather) (void)hipEventDestroy(ev_gather);
+  g500::return_gate(this);
 }
