@@ -54,8 +54,6 @@
 // synchronisations per level.
 #include <algorithm>
 #include <chrono>
-#include <map>
-#include <mutex>
 #include <cmath>
 #include <deque>
 #include <cstdlib>
@@ -1101,27 +1099,14 @@ __device__ inline void queue_wave_span(int64_t words, int64_t* a, int64_t* e) {
   *e = min(*a + per_wave, b1);
 }
 
-// The gather's gate: a value in signal memory that a value wait on the
-// gather stream compares; released to system scope so the wait reads it
-// from memory.
-__device__ inline void open_gate(unsigned long long* gate, unsigned long long no) {
-  __hip_atomic_store(gate, no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
 __global__ __launch_bounds__(kT) void k_queue_count(const uint64_t* __restrict__ cur,
                                                    const uint64_t* __restrict__ prev,
                                                    int64_t words, uint32_t* __restrict__ wcnt,
                                                    uint64_t* __restrict__ snap,
-                                                   LevelSync* ls, uint32_t code,
-                                                   unsigned long long* gate,
-                                                   unsigned long long gate_no) {
-  // a conversion opens the gather's gate whether it runs or not (the levels
-  // before it have ended; the gather checks snap_tag)
-  if (gate && blockIdx.x == 0 && threadIdx.x == 0) {
-    if (step_is_next(ls, code)) ls->snap_tag = code;
-    open_gate(gate, gate_no);
-  }
+                                                   LevelSync* ls, uint32_t code) {
   if (!step_is_next(ls, code)) return;
+  // the gather queued for this conversion checks the tag
+  if (snap && blockIdx.x == 0 && threadIdx.x == 0) ls->snap_tag = code;
   int64_t a, e;
   queue_wave_span(words, &a, &e);
   uint64_t c = 0;
@@ -1373,38 +1358,6 @@ unsigned grid_for(int64_t work_items, int64_t per_block) {
 
 }  // namespace
 
-// Gates live in signal memory, which is not freed once a wait has read it:
-// engines return theirs to a per-device pool for the next engine.
-namespace {
-std::mutex g_gate_mu;
-std::map<int, std::vector<unsigned long long*>> g_gate_pool;
-}  // namespace
-
-void take_gate(g500_bfs* b) {
-  const int dev = b->g->device;
-  unsigned long long* p = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_gate_mu);
-    auto& pool = g_gate_pool[dev];
-    if (!pool.empty()) {
-      p = pool.back();
-      pool.pop_back();
-    }
-  }
-  if (!p)
-    G500_CHECK_HIP(hipExtMallocWithFlags((void**)&p, sizeof(unsigned long long),
-                                         hipMallocSignalMemory));
-  G500_CHECK_HIP(hipMemset(p, 0, sizeof(unsigned long long)));
-  b->gate_conv = p;
-  b->gate_device = dev;
-}
-
-void return_gate(g500_bfs* b) {
-  std::lock_guard<std::mutex> lk(g_gate_mu);
-  if (b->gate_conv) g_gate_pool[b->gate_device].push_back(b->gate_conv);
-  b->gate_conv = nullptr;
-}
-
 void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   g500_graph* g = b->g;
   b->words = ceil_div(std::max<int64_t>(g->n_c, 1), 64);
@@ -1452,7 +1405,7 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
-  take_gate(b);
+  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_snap, hipEventDisableTiming));
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_gather, hipEventDisableTiming));
   G500_CHECK_HIP(hipEventCreate(&b->ev_g0));
   G500_CHECK_HIP(hipEventCreate(&b->ev_g1));
@@ -1597,7 +1550,7 @@ hipEvent_t pool_event(std::vector<hipEvent_t>& pool, size_t i) {
 // no bottom-up level uses (a top-down level run as a sweep refills it from
 // the bitmap afterwards); a top-down level swaps the queues; the conversion
 // at the switch to top-down writes the queue and the gather's bitmap
-// snapshot and opens the gather's gate with its report number.
+// snapshot (ev_snap is recorded as soon as that is written).
 Step launch_step(g500_bfs* b, uint32_t code, const Bufs& in, size_t* nlev_ev,
                  size_t* nsweep_ev) {
   g500_graph* g = b->g;
@@ -1642,9 +1595,9 @@ Step launch_step(g500_bfs* b, uint32_t code, const Bufs& in, size_t* nlev_ev,
                       uint64_t* snap) {
     const dim3 grid((unsigned)std::min<int64_t>(kQueueGrid, ceil_div(b->words, (int64_t)kT)));
     hipLaunchKernelGGL(k_queue_count, grid, dim3(kT), 0, s, cur, prev, b->words, b->qcnt.get(),
-                       snap, ls, code, snap ? b->gate_conv : nullptr,
-                       (unsigned long long)st.seq);
+                       snap, ls, code);
     G500_CHECK_LAUNCH();
+    if (snap) G500_CHECK_HIP(hipEventRecord(b->ev_snap, s));
     hipLaunchKernelGGL(k_queue_write, grid, dim3(kT), 0, s, cur, prev, b->words,
                        g->deg_new.get(), b->qcnt.get(), out, b->qedges.get(), ls, code);
     G500_CHECK_LAUNCH();
@@ -1818,11 +1771,11 @@ struct EndGuess {
 // The engine stream waits until every gather queued so far has ended.
 void wait_gathers(g500_bfs* b) { G500_CHECK_HIP(hipStreamWaitEvent(b->stream, b->ev_gather, 0)); }
 
-// The gather on gstream, behind the conversion with report number conv_seq
-// (snap_code: its code when it is queued on a guess); ev_gather follows it.
-void queue_gather(g500_bfs* b, int64_t root, unsigned long long conv_seq, uint32_t snap_code) {
-  G500_CHECK_HIP(hipStreamWaitValue64(b->gstream, b->gate_conv, conv_seq, hipStreamWaitValueGte,
-                                      ~0ull));
+// The gather on gstream behind the snapshot of the conversion queued last
+// (ev_snap; snap_code: its code when it is queued on a guess); ev_gather
+// follows it.
+void queue_gather(g500_bfs* b, int64_t root, uint32_t snap_code) {
+  G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
   launch_gather(b, root, b->gstream, snap_code);
   G500_CHECK_HIP(hipEventRecord(b->ev_gather, b->gstream));
 }
@@ -1880,11 +1833,9 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
       tr.push_back(t);
     }
     pend.push_back(launch_step(b, next, last.after, nlev_ev, nsweep_ev));
-    // (only behind a list level: the gate's wait polls until the conversion
-    // starts, and a list level before the switch is short)
-    if (step_mode(next) == kModeConv && !eg->gather_confirmed &&
-        step_mode(last.code) == kModeBUList) {
-      queue_gather(b, eg->root, pend.back().seq, next);
+    if (step_mode(next) == kModeConv && !eg->gather_confirmed) {
+      // ev_snap was just recorded behind this conversion's snapshot
+      queue_gather(b, eg->root, next);
       eg->conv_seq = pend.back().seq;
     }
   }
@@ -2084,11 +2035,12 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
     if (step_mode(next) == kModeConv) {
       // switching to top-down: the remaining top-down levels are short and
       // latency-bound, so the pred gather of every vertex reached so far
-      // starts on the second stream as soon as the conversion starts (its
-      // gate); the vertices the last levels reach are patched in at the end
+      // starts on the second stream once the conversion has written the
+      // bitmap snapshot (ev_snap, recorded when the conversion was queued);
+      // the vertices the last levels reach are patched in at the end
       // (k_pred_fixup).  A conversion queued on a guess has its gather
       // queued behind it already.
-      if (eg.conv_seq != pend.front().seq) queue_gather(b, root, pend.front().seq, 0);
+      if (eg.conv_seq != pend.front().seq) queue_gather(b, root, 0);
       early_gather = true;
       eg.gather_confirmed = true;
     }
@@ -2215,6 +2167,6 @@ g500_bfs::~g500_bfs() {
   if (gstream) (void)hipStreamDestroy(gstream);
   for (auto e : level_events) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
+  if (ev_snap) (void)hipEventDestroy(ev_snap);
   if (ev_gather) (void)hipEventDestroy(ev_gather);
-  g500::return_gate(this);
 }

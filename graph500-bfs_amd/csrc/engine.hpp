@@ -95,17 +95,12 @@ struct g500_bfs {
   // it covers (copied by the conversion's count pass), the fixup adds the rest
   hipStream_t gstream = nullptr;
   g500::DevBuf<uint64_t> vis_snap;
-  // the gather's gate, in signal memory (never freed, as memory a wait has
-  // read; engines pass it on through a pool): report number of the last
-  // conversion whose count pass has started.  gstream waits on it before a
-  // gather with a value wait, which wakes 2-3 us after the write (an event
-  // wait: ~11 us) but polls from a kernel that slows the kernels beside it,
-  // so it is queued only just before the conversion.  ev_gather follows the
-  // last gather queued (the engine stream waits on it before the fixup and
-  // the next run: no poll beside the gather).
-  unsigned long long* gate_conv = nullptr;
-  int gate_device = -1;
-  hipEvent_t ev_gather = nullptr;
+  // ev_snap follows the last conversion's count pass (the gather waits on
+  // it), ev_gather the last gather queued (the engine stream waits on it
+  // before the fixup and the next run).  Events, not value waits: a value
+  // wait polls from a kernel, which never ends where the kernels of a device
+  // run one at a time (as under a counter profiler)
+  hipEvent_t ev_snap = nullptr, ev_gather = nullptr;
   g500::DevBuf<uint64_t> lpart;   // per-block totals of a level's kernels
   g500::DevBuf<uint32_t> qcnt;    // per-wave counts of the bitmap -> queue conversion
   g500::DevBuf<uint64_t> qedges;  // its per-block degree sums
@@ -145,6 +140,4 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
 void bfs_validate(g500_bfs* b, int64_t root, const int64_t* host_pred,
                   int64_t* te, int64_t* errs, int* ok);
 void bfs_levels(g500_bfs* b, uint16_t* out);
-void take_gate(g500_bfs* b);    // from the device's pool, cleared
-void return_gate(g500_bfs* b);  // back to the pool
 }  // namespace g500
