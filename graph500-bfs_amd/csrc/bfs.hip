@@ -1443,6 +1443,7 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
   ensure_host_orig2new(g);
   if (const char* ea = getenv("G500_EXEC_ALPHA")) b->exec_alpha = atof(ea);
   if (const char* sp = getenv("G500_SPECULATE")) b->spec_depth = std::max(0, atoi(sp));
+  if (const char* gw = getenv("G500_GUESS")) b->guess_wrong = std::strcmp(gw, "wrong") == 0;
   if (g->corebfs && !g->core_reset_done) {
     // corebfs_index::reset_unreachable: one BFS over the core from its
     // highest-degree vertex decides which forest parents survive
@@ -1789,6 +1790,21 @@ void queue_fixup(g500_bfs* b, const uint64_t* cur, uint32_t done_code) {
   G500_CHECK_LAUNCH();
 }
 
+// The test hook's wrong guess for a guessed step `code`: a guessed end of the
+// run becomes a top-down level, a conversion a list level, a bottom-up level
+// a conversion (its gather is queued too) and a top-down level the end of
+// the run (the fixup is queued too, once the gather runs).
+uint32_t wrong_guess(uint32_t code) {
+  const int l = step_level(code);
+  switch (step_mode(code)) {
+    case kModeDone: return step_code(l, kModeTD);
+    case kModeConv: return step_code(l, kModeBUList);
+    case kModeBUList:
+    case kModeBUSweep: return step_code(l, kModeConv);
+    default: return step_code(l, kModeDone);
+  }
+}
+
 // Queues guessed steps behind the queued ones until `depth` are queued:
 // the steps queued after the first one (whose predecessor has reported)
 // contribute their expected totals to the guess of the next one.  A guessed
@@ -1813,7 +1829,8 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
   }
   while (pend.size() < depth) {
     const Step& last = pend.back();
-    const uint32_t next = predict_next(last, h, tr);
+    uint32_t next = predict_next(last, h, tr);
+    if (b->guess_wrong) next = wrong_guess(next);
     if (step_mode(next) == kModeDone) {
       if (eg->gather_confirmed && eg->fixup_seq != last.seq) {
         queue_fixup(b, last.after.vis, next);

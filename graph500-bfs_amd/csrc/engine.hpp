@@ -83,6 +83,10 @@ struct g500_bfs {
   // guessed steps queued behind the running one (G500_SPECULATE=<n>; 0: the
   // host waits for every step's report before it launches the next)
   int spec_depth = 1;
+  // test hook (G500_GUESS=wrong): every guess is turned into a wrong one of
+  // another kind, so the guarded launches, the guessed gather and the
+  // guessed fixup run against a loop state that does not name them
+  bool guess_wrong = false;
   int64_t unreach = 0;  // vertices with edges the last run left unreached
 
   // validation scratch

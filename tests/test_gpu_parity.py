@@ -262,7 +262,7 @@ def test_roots_without_edges(G):
 @pytest.mark.parametrize("exec_alpha", [0.0, 1e9, None])
 def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
     """Top-down levels run as top-down (0), as bottom-up sweeps (1e9) or as
-    the default decides (over m/15 edges: bottom-up): the levels, the
+    the default decides (over m/25 edges: bottom-up): the levels, the
     direction schedule and auto_tuning_data stay identical."""
     print(f"\n[exec] alpha_exec={exec_alpha}", flush=True)
     g = G.Graph.generate(16).construct()
@@ -287,10 +287,47 @@ def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
             m = g.info.m
             for prev, t in zip(trace, trace[1:]):
                 if t["top_down"]:
-                    heavy = prev["top_down"] and prev["nq_edges"] > m / 15
+                    heavy = prev["top_down"] and prev["nq_edges"] > m // 25
                     assert t["exec_bottom_up"] == int(heavy), (r, t)
         else:
             assert all(t["exec_bottom_up"] == 1 for t in trace[1:])
+
+
+@pytest.mark.parametrize("depth,guess", [("0", ""), ("1", "wrong"), ("2", "wrong"), ("3", "")])
+def test_step_guesses_never_change_results(G, depth, guess, golden, monkeypatch):
+    """The host queues guessed steps (and, on guesses, the pred gather behind
+    a conversion and the fixup behind the last step) before the step ahead
+    has reported; every kernel checks the loop state.  With no guesses, with
+    deeper speculation, and with every guess turned into a wrong one of
+    another kind (G500_GUESS=wrong: a bottom-up level guessed to be followed
+    by the conversion, a top-down level by the end of the run, ...), the
+    levels, the direction schedule, auto_tuning_data and the validated tree
+    stay those of the oracle; wrong guesses are counted."""
+    print(f"\n[guess] depth={depth} guess={guess!r}", flush=True)
+    monkeypatch.setenv("G500_SPECULATE", depth)
+    monkeypatch.setenv("G500_GUESS", guess)
+    g = G.Graph.generate(16).construct()
+    o0, o1 = O.generate_edges(16)
+    og = O.Graph(o0, o1)
+    bfs = G.Bfs(g)  # reads both variables
+    wrong = 0
+    for r in g.find_roots(8):
+        r = int(r)
+        at = bfs.run(r)
+        assert np.array_equal(bfs.levels(), _oracle_levels(og, r)), r
+        ok, te, errs = bfs.validate(r)
+        assert ok and te == golden["pf_nedge"][16], (r, errs)
+        _, at_o, dirs_o = og.do_bfs(r)
+        st, trace = bfs.stats()
+        assert [t["top_down"] for t in trace] == dirs_o.tolist(), r
+        assert at.tolist() == at_o.tolist(), r
+        wrong += st.mispredicted
+    if depth == "0":
+        assert wrong == 0
+    if guess == "wrong":
+        assert wrong >= 8  # at least one per run
+    bfs.destroy()
+    g.free()
 
 
 def _ref_case(scale, corebfs):
