@@ -179,7 +179,11 @@ int g500_bfs_create(g500_graph* g, g500_bfs** out);
 /* The same with prepare_bfs's edgefactor, alpha and beta (mpi/bfs.hpp:141-151),
  * which the CoreBFS reachability BFS of a graph's first engine runs with
  * (reset_unreachable, mpi/corebfs_adaptor.hpp:228-249); g500_bfs_create uses
- * the reference defaults 16, 10, 14. */
+ * the reference defaults 16, 10, 14.  Both read, at creation, G500_EXEC_ALPHA
+ * (see g500_bfs_set_exec_alpha), G500_SPECULATE (guessed steps the host
+ * queues ahead of the running one, default 1, 0 = none) and G500_GUESS
+ * ("wrong": a test setting that makes every guess wrong; results stay the
+ * same, runs get slower). */
 int g500_bfs_create_ex(g500_graph* g, int edgefactor, double alpha, double beta,
                        g500_bfs** out);
 
