@@ -284,28 +284,53 @@ __global__ __launch_bounds__(kT) void k_ms_prep(MsLevel a, int64_t bm_words) {
       }
 }
 
-// Top-down: push f to w (lanes not yet seen at w).  The ORs return nothing,
-// so a push waits on no atomic: the level's counts (new pairs, frontier
-// degree sum) are taken afterwards from F_{L+1} by k_ms_td_count.  The
-// bitmap word is read beside seen so a vertex already in the next frontier
-// costs no second atomic.
-template <int K>
-__device__ inline void ms_push(const MsLevel& a, uint32_t w, const uint64_t (&f)[K]) {
-  uint64_t sw[K];
-  load_lanes<K>(a.seen + (int64_t)w * K, sw);
-  const uint64_t bit = 1ull << (w & 63);
-  const bool listed = (__hip_atomic_load(&a.fbn[w >> 6], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT) & bit) != 0;
-  bool any = false;
+// Pushes run kPushIlp edges per lane at a time: the targets' seen words and
+// bitmap words are all requested before the first atomic, since a wave
+// waits for its atomics along with its loads (one vector memory counter),
+// so a loop of single pushes pays three round trips per edge.
+constexpr int kPushIlp = 4;
+
+// The row entries [e0 + u*step] (u < kPushIlp) below end: targets, and
+// whether each is in the row.
+__device__ inline void push_targets(const uint32_t* col, int64_t e0, int64_t step, int64_t end,
+                                    uint32_t (&w)[kPushIlp], bool (&ok)[kPushIlp]) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint64_t m = f[k] & ~sw[k];
-    if (m) {
-      atomicOr((unsigned long long*)&a.fn[(int64_t)w * K + k], (unsigned long long)m);
-      any = true;
-    }
+  for (int u = 0; u < kPushIlp; ++u) {
+    const int64_t e = e0 + (int64_t)u * step;
+    ok[u] = e < end;
+    w[u] = ok[u] ? ld_col(col + e) : 0u;
   }
-  if (any && !listed) atomicOr((unsigned long long*)&a.fbn[w >> 6], (unsigned long long)bit);
+}
+
+// Top-down: push f to the targets w (lanes not yet seen there; a target
+// with !ok pushes nothing).  The ORs return nothing, so a push waits on no
+// atomic: the level's counts (new pairs, frontier degree sum) are taken
+// afterwards from F_{L+1} by k_ms_td_count.  The bitmap word is read beside
+// seen so a vertex already in the next frontier costs no second atomic.
+template <int K>
+__device__ inline void ms_push_n(const MsLevel& a, const uint32_t (&w)[kPushIlp],
+                                 const bool (&ok)[kPushIlp], const uint64_t (&f)[K]) {
+  uint64_t sw[kPushIlp][K], bw[kPushIlp];
+#pragma unroll
+  for (int u = 0; u < kPushIlp; ++u) {
+    load_lanes<K>(a.seen + (int64_t)w[u] * K, sw[u]);
+    bw[u] = __hip_atomic_load(&a.fbn[w[u] >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+#pragma unroll
+  for (int u = 0; u < kPushIlp; ++u) {
+    const uint64_t bit = 1ull << (w[u] & 63);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t m = ok[u] ? f[k] & ~sw[u][k] : 0ull;
+      if (m) {
+        atomicOr((unsigned long long*)&a.fn[(int64_t)w[u] * K + k], (unsigned long long)m);
+        any = true;
+      }
+    }
+    if (any && !(bw[u] & bit))
+      atomicOr((unsigned long long*)&a.fbn[w[u] >> 6], (unsigned long long)bit);
+  }
 }
 
 // Top-down over new IDs [vb, ve), G lanes per vertex.
@@ -320,7 +345,12 @@ __global__ __launch_bounds__(kT) void k_ms_td(MsLevel a, int64_t vb, int64_t ve)
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
     const int64_t end = a.row_off[v + 1];
-    for (int64_t e = a.row_off[v] + lig; e < end; e += G) ms_push<K>(a, ld_col(a.col + e), f);
+    for (int64_t e0 = a.row_off[v] + lig; e0 < end; e0 += (int64_t)G * kPushIlp) {
+      uint32_t w[kPushIlp];
+      bool ok[kPushIlp];
+      push_targets(a.col, e0, G, end, w, ok);
+      ms_push_n<K>(a, w, ok, f);
+    }
   }
 }
 
@@ -341,7 +371,12 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
     load_lanes<K>(a.fl + v * K, f);
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e = beg + lane; e < end; e += kWave) ms_push<K>(a, ld_col(a.col + e), f);
+    for (int64_t e0 = beg + lane; e0 < end; e0 += (int64_t)kWave * kPushIlp) {
+      uint32_t w[kPushIlp];
+      bool ok[kPushIlp];
+      push_targets(a.col, e0, kWave, end, w, ok);
+      ms_push_n<K>(a, w, ok, f);
+    }
   }
 }
 
@@ -647,15 +682,26 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_pull_heavy(MsLevel a,
   }
 }
 
+// Pushes without a seen filter (the pulls mask F_{L+1} with seen): the
+// lanes of f missing from each target's word, ORed in; lanes only ever get
+// added, so a stale read can only cause an extra atomic.
 template <int K>
-__device__ inline void hyb_push_one(const MsLevel& a, uint32_t w, const uint64_t (&f)[K]) {
+__device__ inline void hyb_push_n(const MsLevel& a, const uint32_t (&w)[kPushIlp],
+                                  const bool (&ok)[kPushIlp], const uint64_t (&f)[K]) {
+  uint64_t x[kPushIlp][K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    unsigned long long* p = (unsigned long long*)&a.fn[(int64_t)w * K + k];
-    // lanes only ever get added, so a stale read can only cause an extra atomic
-    const uint64_t m = f[k] & ~(uint64_t)__atomic_load_n(p, __ATOMIC_RELAXED);
-    if (m) atomicOr(p, (unsigned long long)m);
-  }
+  for (int u = 0; u < kPushIlp; ++u)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      x[u][k] = (uint64_t)__atomic_load_n((unsigned long long*)&a.fn[(int64_t)w[u] * K + k],
+                                          __ATOMIC_RELAXED);
+#pragma unroll
+  for (int u = 0; u < kPushIlp; ++u)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t m = ok[u] ? f[k] & ~x[u][k] : 0ull;
+      if (m) atomicOr((unsigned long long*)&a.fn[(int64_t)w[u] * K + k], (unsigned long long)m);
+    }
 }
 
 // Push side of the hybrid: frontier vertices in [max(vb, hot), ve), G lanes each.
@@ -670,7 +716,12 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push(MsLevel a, int64_t vb, int64
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
     const int64_t end = a.row_off[v + 1];
-    for (int64_t e = a.row_off[v] + lig; e < end; e += G) hyb_push_one<K>(a, ld_col(a.col + e), f);
+    for (int64_t e0 = a.row_off[v] + lig; e0 < end; e0 += (int64_t)G * kPushIlp) {
+      uint32_t w[kPushIlp];
+      bool ok[kPushIlp];
+      push_targets(a.col, e0, G, end, w, ok);
+      hyb_push_n<K>(a, w, ok, f);
+    }
   }
 }
 
@@ -692,7 +743,12 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_push_heavy(MsLevel a,
     load_lanes<K>(a.fl + v * K, f);
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e = beg + lane; e < end; e += kWave) hyb_push_one<K>(a, ld_col(a.col + e), f);
+    for (int64_t e0 = beg + lane; e0 < end; e0 += (int64_t)kWave * kPushIlp) {
+      uint32_t w[kPushIlp];
+      bool ok[kPushIlp];
+      push_targets(a.col, e0, kWave, end, w, ok);
+      hyb_push_n<K>(a, w, ok, f);
+    }
   }
 }
 
@@ -775,7 +831,12 @@ __global__ __launch_bounds__(kT) void k_ms_split_push(MsLevel a, int64_t vb, int
     }
     if (!any) continue;
     const int64_t end = a.row_off[v + 1];
-    for (int64_t e = a.row_off[v] + lig; e < end; e += G) hyb_push_one<K>(a, ld_col(a.col + e), f);
+    for (int64_t e0 = a.row_off[v] + lig; e0 < end; e0 += (int64_t)G * kPushIlp) {
+      uint32_t w[kPushIlp];
+      bool ok[kPushIlp];
+      push_targets(a.col, e0, G, end, w, ok);
+      hyb_push_n<K>(a, w, ok, f);
+    }
   }
 }
 
@@ -811,7 +872,12 @@ __global__ __launch_bounds__(kT) void k_ms_split_push_heavy(MsLevel a,
     if (!any) continue;
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e = beg + lane; e < end; e += kWave) hyb_push_one<K>(a, ld_col(a.col + e), f);
+    for (int64_t e0 = beg + lane; e0 < end; e0 += (int64_t)kWave * kPushIlp) {
+      uint32_t w[kPushIlp];
+      bool ok[kPushIlp];
+      push_targets(a.col, e0, kWave, end, w, ok);
+      hyb_push_n<K>(a, w, ok, f);
+    }
   }
 }
 
