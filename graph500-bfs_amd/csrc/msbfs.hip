@@ -922,7 +922,10 @@ __global__ __launch_bounds__(kT) void k_ms_commit(MsLevel a, int want) {
 
 // Bottom-up over new IDs [vb, ve), G lanes per vertex, U neighbours per lane
 // per round; a vertex stops at the first round after which every lane it
-// lacks has a neighbour in that lane's frontier.
+// lacks has a neighbour in that lane's frontier.  The neighbours' lane words
+// are read without a frontier-bitmap test first (a word outside the
+// frontier is zero): on the levels that pull, most neighbours are in the
+// frontier, and the test cost a round trip per round.
 template <int K, int G, int U>
 __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve) {
   __shared__ uint64_t red[kWavesPerBlock];
@@ -981,7 +984,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu(MsLevel a, int64_t vb, int64_t ve)
           }
 #pragma unroll
           for (int j = 0; j < U; ++j) {
-            if (u[j] != 0xFFFFFFFFu && in_bitmap(a.fb, u[j])) {
+            if (u[j] != 0xFFFFFFFFu) {
               uint64_t x[K];
               load_lanes<K>(a.fl + (int64_t)u[j] * K, x);
 #pragma unroll
@@ -1085,7 +1088,7 @@ __global__ __launch_bounds__(kT) void k_ms_bu_heavy(MsLevel a,
       }
 #pragma unroll
       for (int j = 0; j < U; ++j) {
-        if (u[j] != 0xFFFFFFFFu && in_bitmap(a.fb, u[j])) {
+        if (u[j] != 0xFFFFFFFFu) {
           uint64_t x[K];
           load_lanes<K>(a.fl + (int64_t)u[j] * K, x);
 #pragma unroll
