@@ -1241,17 +1241,62 @@ __global__ __launch_bounds__(kT) void k_ms_output(const int64_t* __restrict__ d_
 // word at a time, writes the 64 levels of each vertex into an LDS tile
 // [64 roots][kT vertices] and stores each root's kT-byte row segment with
 // 16-byte stores.
+//
+// With one or two root words and at most kPlaneLevels levels, k_ms_planes
+// first walks the vertices in new-ID order (each level frontier read once,
+// coalesced) and stores every vertex's levels as four bit planes, contiguous
+// per vertex (4 K words: 32 or 64 bytes; 15 = not reached); the output then
+// reads that one run per vertex instead of a word per level at a random
+// place in each frontier (the level reads fetched ~4x their bytes).
+constexpr int kPlaneLevels = 15;
+
+template <int K>
+__global__ __launch_bounds__(kT) void k_ms_planes(const uint64_t* const* __restrict__ fptr,
+                                                 const unsigned long long* __restrict__ cnt,
+                                                 int cap, int64_t n_c,
+                                                 uint64_t* __restrict__ planes) {
+  const int nlev = ms_levels(cnt, cap);
+  if (nlev < 0 || nlev > kPlaneLevels) return;
+  for (int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x; v < n_c;
+       v += (int64_t)gridDim.x * kT) {
+    uint64_t P[4][K], R[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      R[k] = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) P[b][k] = 0;
+    }
+    for (int l = 0; l < nlev; ++l) {
+      uint64_t f[K];
+      load_lanes<K>(fptr[l] + v * K, f);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        R[k] |= f[k];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if ((l >> b) & 1) P[b][k] |= f[k];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int k = 0; k < K; ++k) planes[v * 4 * K + b * K + k] = P[b][k] | ~R[k];
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(kT) void k_ms_output_tile(const int64_t* __restrict__ d_roots,
                                                       const uint32_t* __restrict__ o2n,
                                                       int64_t n, int64_t n_c,
                                                       const uint64_t* const* __restrict__ fptr,
                                                       const unsigned long long* __restrict__ cnt,
-                                                      int cap, uint8_t* __restrict__ out) {
+                                                      int cap, const uint64_t* __restrict__ planes,
+                                                      uint8_t* __restrict__ out) {
   __shared__ __align__(16) uint8_t tile[64 * kT];
   const int nroots = (int)d_roots[0];
   const int nlev = ms_levels(cnt, cap);
   if (nlev < 0) return;
+  if (nlev > kPlaneLevels) planes = nullptr;  // k_ms_planes left them
   const int t = threadIdx.x;
   for (int64_t base = (int64_t)blockIdx.x * kT; base < n; base += (int64_t)gridDim.x * kT) {
     const int64_t v = o2n[base + t];
@@ -1262,7 +1307,17 @@ __global__ __launch_bounds__(kT) void k_ms_output_tile(const int64_t* __restrict
 #pragma unroll
       for (int b = 0; b < 8; ++b) P[b][k] = 0;
     }
-    if (v < n_c) {
+    if (v < n_c && planes) {
+      // four planes; 15 (all four set) marks a lane that never reached v,
+      // whose upper planes are set too (0xFF)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) load_lanes<K>(planes + v * 4 * K + b * K, P[b]);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint64_t none = P[0][k] & P[1][k] & P[2][k] & P[3][k];
+        R[k] = ~none;
+      }
+    } else if (v < n_c) {
       int l = 0;
       for (; l + 2 <= nlev; l += 2) {
         uint64_t f0[K], f1[K];
@@ -1634,9 +1689,19 @@ void launch_output(const g500_msbfs* b, hipStream_t s) {
   } else if (g->n % kT == 0) {
     const unsigned grid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>(g->n / kT, (int64_t)num_cus(g->device) * 8));
+    const uint64_t* planes = nullptr;
+    if (K <= 2 && b->planes.get()) {
+      planes = b->planes.get();
+      const unsigned pgrid = (unsigned)std::max<int64_t>(
+          1, std::min<int64_t>(ceil_div(g->n_c, kT), (int64_t)num_cus(g->device) * 16));
+      hipLaunchKernelGGL((k_ms_planes<K>), dim3(pgrid), dim3(kT), 0, s,
+                         (const uint64_t* const*)b->fptr.get(), b->cnt.get(), cap, g->n_c,
+                         b->planes.get());
+      G500_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL((k_ms_output_tile<K>), dim3(grid), dim3(kT), 0, s, b->roots.get(),
                        g->orig2new.get(), g->n, g->n_c, (const uint64_t* const*)b->fptr.get(),
-                       b->cnt.get(), cap, b->out.get());
+                       b->cnt.get(), cap, planes, b->out.get());
   } else {
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(g->n, kT), 65536);
     hipLaunchKernelGGL((k_ms_output<K>), dim3(grid), dim3(kT), 0, s, b->roots.get(),
@@ -1816,7 +1881,10 @@ int64_t msbfs_footprint(const g500_graph* g, int max_roots, bool level_arrays) {
   const int64_t bw = std::max<int64_t>(1, ceil_div(g->n_c, 64));
   int64_t bytes = nc * kmax * 8;                      // seen
   bytes += (int64_t)kInitialLevels * nc * kmax * 8;   // level frontiers
-  if (level_arrays) bytes += (int64_t)max_roots * g->n;  // level arrays
+  if (level_arrays) {
+    bytes += (int64_t)max_roots * g->n;                 // level arrays
+    bytes += nc * 4 * std::min<int64_t>(kmax, 2) * 8;  // their bit planes (K <= 2)
+  }
   bytes += g->n * 4;                                  // cnt0
   bytes += 2 * bw * 8;                                // frontier bitmaps
   bytes += 4 * (kMaxLevels + 2) * 8 + 8 * 1024;       // counters, roots, te, tables
@@ -1843,7 +1911,10 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
   G500_CHECK_HIP(hipMemset(b->split_mask.get(), 0, 4 * sizeof(uint64_t)));
   b->roots.alloc(G500_MSBFS_MAX_ROOTS + 1);
   b->level_arrays = level_arrays;
-  if (level_arrays) b->out.alloc((size_t)max_roots * (size_t)g->n);
+  if (level_arrays) {
+    b->out.alloc((size_t)max_roots * (size_t)g->n);
+    b->planes.alloc((size_t)nc * 4 * std::min(b->kmax, 2));
+  }
   b->fb[0].alloc((size_t)bitmap_words(b));
   b->fb[1].alloc((size_t)bitmap_words(b));
   b->fe.alloc(kMaxLevels + 2);

@@ -35,6 +35,7 @@ struct g500_msbfs {
   int64_t* h_roots = nullptr;               // pinned staging for the roots
   unsigned long long* h_cnt = nullptr;      // pinned copy of cnt after a run
   g500::DevBuf<uint8_t> out;                // levels [max_roots][n] by original ID
+  g500::DevBuf<uint64_t> planes;            // with out: levels as 4 bit planes per vertex (K <= 2)
   bool level_arrays = true;                 // false: point lookups only (no `out`)
   g500::DevBuf<uint32_t> stream_tile_v0;    // streaming pull: first vertex per 1024 edges
   g500::DevBuf<int64_t> gather_pairs;       // point-lookup staging

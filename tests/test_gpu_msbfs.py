@@ -287,8 +287,9 @@ def test_engine_without_level_arrays(G, s12):
     print("\n[msbfs] no level arrays", flush=True)
     g, og = s12
     roots = _mixed_roots(g, og, 90, seed=17)
-    assert G.msbfs_footprint(g, 90, level_arrays=False) + 90 * g.info.n == \
-        G.msbfs_footprint(g, 90)
+    # the level arrays, and their bit planes (four per vertex, two lane words)
+    assert G.msbfs_footprint(g, 90, level_arrays=False) + 90 * g.info.n + \
+        max(g.info.n_c, 1) * 4 * 2 * 8 == G.msbfs_footprint(g, 90)
     mb = G.MultiBfs(g, max_roots=90, level_arrays=False)
     mb.run(roots)
     ref = G.MultiBfs(g, max_roots=90)
