@@ -215,6 +215,49 @@ def test_depth_beyond_captured_levels(G, use_graph):
         assert list(mb.te()) == [199] * 4
 
 
+def _host_levels(n, src, dst, root):
+    adj = [[] for _ in range(n)]
+    for a, b in zip(src.tolist(), dst.tolist()):
+        adj[a].append(b)
+        adj[b].append(a)
+    lev = np.full(n, 0xFF, np.uint8)
+    lev[root] = 0
+    frontier = [root]
+    d = 0
+    while frontier:
+        d += 1
+        nxt = []
+        for u in frontier:
+            for w in adj[u]:
+                if lev[w] == 0xFF:
+                    lev[w] = d
+                    nxt.append(w)
+        frontier = nxt
+    return lev
+
+
+@pytest.mark.parametrize("plen", [15, 16])
+@pytest.mark.parametrize("nroots", [3, 65])
+def test_level_planes_at_their_depth_limit(G, plen, nroots):
+    """The level arrays of one or two root words go through four bit planes
+    per vertex while a batch has at most 15 levels (value 15 marks an
+    unreached lane), else straight from the level frontiers: a path of 15
+    or 16 vertices with 285 leaves on its second vertex (n = 512) gives 15
+    or 16 levels from the path's ends."""
+    print(f"\n[msbfs] planes path {plen} roots {nroots}", flush=True)
+    src = np.concatenate([np.arange(plen - 1), np.ones(300 - plen, np.int64)])
+    dst = np.concatenate([np.arange(1, plen), np.arange(plen, 300)])
+    g = G.Graph.from_edges(src.astype(np.int64), dst.astype(np.int64)).construct()
+    n = g.info.n
+    assert n % 256 == 0
+    mb = G.MultiBfs(g, max_roots=128)
+    ends = [0, plen - 1, plen // 2, plen, 299]
+    roots = np.array([ends[i % len(ends)] for i in range(nroots)], np.int64)
+    mb.run(roots)
+    for i in sorted({0, 1, 2, nroots - 1}):
+        assert np.array_equal(mb.levels(i), _host_levels(n, src, dst, int(roots[i]))), i
+
+
 def test_depth_limit_is_an_error(G):
     print("\n[msbfs] path 300: depth 299 > 254", flush=True)
     g = _path_graph(G, 300)
