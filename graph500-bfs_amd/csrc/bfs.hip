@@ -444,7 +444,11 @@ __global__ __launch_bounds__(kT) void k_td_expand(
   __shared__ uint64_t win[kTdWin];
   __shared__ int64_t wrb[kTdWin];
   __shared__ uint32_t wpu[kTdWin];
-  __shared__ uint32_t lq[kTdTile];
+  // winners of the block's tiles, flushed to the next queue with one global
+  // reservation once half a tile's worth has gathered (and after the last
+  // tile): a reservation per tile serialized ~12 ns each on one counter,
+  // which on levels of 10^8 edges (5*10^4 tiles) was most of their time
+  __shared__ uint32_t lq[kTdTile + kTdTile / 2];
   __shared__ uint64_t red[kWavesPerBlock];
   __shared__ int64_t s_lo, s_hi;
   __shared__ uint32_t lq_n;
@@ -485,6 +489,7 @@ __global__ __launch_bounds__(kT) void k_td_expand(
     E = dsum[qlen];
   }
   uint64_t my_edges = 0;
+  if (t == 0) lq_n = 0;
   for (uint64_t e0 = (uint64_t)blockIdx.x * kTdTile; e0 < E;
        e0 += (uint64_t)gridDim.x * kTdTile) {
     const uint64_t e1 = (e0 + kTdTile < E) ? e0 + kTdTile : E;
@@ -494,7 +499,6 @@ __global__ __launch_bounds__(kT) void k_td_expand(
       const uint64_t tile = e0 / kTdTile;
       s_lo = small ? 0 : tile_owner[tile];
       s_hi = small ? qlen - 1 : min((int64_t)tile_owner[tile + 1], qlen - 1);
-      lq_n = 0;
     }
     __syncthreads();
     const int64_t lo = s_lo, hi = s_hi;
@@ -575,10 +579,16 @@ __global__ __launch_bounds__(kT) void k_td_expand(
         if (won[j]) lq[pos++] = v[j];
     }
     __syncthreads();
+    // below kTdTile / 2 left, the next tile's winners (at most kTdTile) fit
     const uint32_t cnt = lq_n;
-    if (t == 0 && cnt) lq_base = atomicAdd(&ls->cnt[0], (unsigned long long)cnt);
-    __syncthreads();
-    for (uint32_t j = t; j < cnt; j += kT) nq[lq_base + j] = lq[j];
+    const bool last = e0 + (uint64_t)gridDim.x * kTdTile >= E;
+    if (cnt >= kTdTile / 2 || (last && cnt)) {
+      if (t == 0) lq_base = atomicAdd(&ls->cnt[0], (unsigned long long)cnt);
+      __syncthreads();
+      for (uint32_t j = t; j < cnt; j += kT) nq[lq_base + j] = lq[j];
+      __syncthreads();
+      if (t == 0) lq_n = 0;
+    }
     __syncthreads();
   }
   const uint64_t s = block_sum(my_edges, red);
