@@ -293,8 +293,10 @@ def test_execution_mode_keeps_reference_schedule(G, exec_alpha, golden):
             assert all(t["exec_bottom_up"] == 1 for t in trace[1:])
 
 
-@pytest.mark.parametrize("depth,guess", [("0", ""), ("1", "wrong"), ("2", "wrong"), ("3", "")])
-def test_step_guesses_never_change_results(G, depth, guess, golden, monkeypatch):
+@pytest.mark.parametrize("depth,guess,corebfs", [("0", "", False), ("1", "wrong", False),
+                                                 ("2", "wrong", False), ("3", "", False),
+                                                 ("1", "wrong", True)])
+def test_step_guesses_never_change_results(G, depth, guess, corebfs, golden, monkeypatch):
     """The host queues guessed steps (and, on guesses, the pred gather behind
     a conversion and the fixup behind the last step) before the step ahead
     has reported; every kernel checks the loop state.  With no guesses, with
@@ -302,11 +304,13 @@ def test_step_guesses_never_change_results(G, depth, guess, golden, monkeypatch)
     another kind (G500_GUESS=wrong: a bottom-up level guessed to be followed
     by the conversion, a top-down level by the end of the run, ...), the
     levels, the direction schedule, auto_tuning_data and the validated tree
-    stay those of the oracle; wrong guesses are counted."""
-    print(f"\n[guess] depth={depth} guess={guess!r}", flush=True)
+    stay those of the oracle; wrong guesses are counted.  With CoreBFS (the
+    gather that also writes the forest's entries) the levels after the depth
+    rebuild and the validated tree are checked."""
+    print(f"\n[guess] depth={depth} guess={guess!r} corebfs={corebfs}", flush=True)
     monkeypatch.setenv("G500_SPECULATE", depth)
     monkeypatch.setenv("G500_GUESS", guess)
-    g = G.Graph.generate(16).construct()
+    g = G.Graph.generate(16).construct(corebfs=corebfs)
     o0, o1 = O.generate_edges(16)
     og = O.Graph(o0, o1)
     bfs = G.Bfs(g)  # reads both variables
@@ -314,14 +318,19 @@ def test_step_guesses_never_change_results(G, depth, guess, golden, monkeypatch)
     for r in g.find_roots(8):
         r = int(r)
         at = bfs.run(r)
-        assert np.array_equal(bfs.levels(), _oracle_levels(og, r)), r
         ok, te, errs = bfs.validate(r)
         assert ok and te == golden["pf_nedge"][16], (r, errs)
-        _, at_o, dirs_o = og.do_bfs(r)
         st, trace = bfs.stats()
+        wrong += st.mispredicted
+        if corebfs:
+            lev = bfs.levels().astype(np.int64)
+            lev[lev == 0xFFFF] = -1
+            assert np.array_equal(lev, og.bfs(r)), r
+            continue
+        assert np.array_equal(bfs.levels(), _oracle_levels(og, r)), r
+        _, at_o, dirs_o = og.do_bfs(r)
         assert [t["top_down"] for t in trace] == dirs_o.tolist(), r
         assert at.tolist() == at_o.tolist(), r
-        wrong += st.mispredicted
     if depth == "0":
         assert wrong == 0
     if guess == "wrong":
