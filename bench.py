@@ -335,8 +335,8 @@ def scale24_leg(g500bfs, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not built.", flush=True)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the engine "
+              "works on one GPU's graph; sharding over several GPUs is not built.", flush=True)
         sys.exit(2)
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be >= 1 and --warmup >= 0", flush=True)
