@@ -1595,7 +1595,11 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
   launch_td<K, 1>(b, a, m, n_c, s);
   hipLaunchKernelGGL((k_ms_td_count<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a);
   G500_CHECK_LAUNCH();
-  if (b->g->m > 0) {
+  // Every kernel below is queued for every level and returns at once unless
+  // the level's mode (decided on the device) is its own; such an empty
+  // kernel still costs ~4-5 us.  Streaming levels need stream_frac > 0, and
+  // split levels split_bp > 0, so without them their kernels are left out.
+  if (b->g->m > 0 && b->tuning.stream_frac > 0) {
     const int64_t ntiles = ceil_div(b->g->m, StreamTile<K>::kTile);
     const unsigned sgrid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>(ntiles, (int64_t)num_cus(b->g->device) * b->tuning.blocks_per_cu));
@@ -1637,21 +1641,23 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
   }
   // split level: the pushing lanes, then their pushes, before the pulls
   // (which take the pushed lanes into F_{L+1} with their own)
-  hipLaunchKernelGGL((k_ms_lane_mask<K>), dim3(1), dim3(kT), 0, s, a, kLaneParts);
-  G500_CHECK_LAUNCH();
-  if (b->n_items > 0) {
-    hipLaunchKernelGGL((k_ms_split_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
-                       b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
+  if (a.split_thr > 0 && L > 0) {
+    hipLaunchKernelGGL((k_ms_lane_mask<K>), dim3(1), dim3(kT), 0, s, a, kLaneParts);
     G500_CHECK_LAUNCH();
+    if (b->n_items > 0) {
+      hipLaunchKernelGGL((k_ms_split_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
+                         b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
+      G500_CHECK_LAUNCH();
+    }
+    switch (b->tuning.medium_group) {
+      case 2: launch_split_push<K, 2>(b, a, h, m, s); break;
+      case 4: launch_split_push<K, 4>(b, a, h, m, s); break;
+      case 8: launch_split_push<K, 8>(b, a, h, m, s); break;
+      case 16: launch_split_push<K, 16>(b, a, h, m, s); break;
+      default: launch_split_push<K, 32>(b, a, h, m, s); break;
+    }
+    launch_split_push<K, 1>(b, a, m, n_c, s);
   }
-  switch (b->tuning.medium_group) {
-    case 2: launch_split_push<K, 2>(b, a, h, m, s); break;
-    case 4: launch_split_push<K, 4>(b, a, h, m, s); break;
-    case 8: launch_split_push<K, 8>(b, a, h, m, s); break;
-    case 16: launch_split_push<K, 16>(b, a, h, m, s); break;
-    default: launch_split_push<K, 32>(b, a, h, m, s); break;
-  }
-  launch_split_push<K, 1>(b, a, m, n_c, s);
   if (b->n_items > 0) {
     hipLaunchKernelGGL((k_ms_bu_heavy<K, 4>), dim3(hgrid), dim3(kT), 0, s, a,
                        b->item_start.get(), b->item_vertex.get(), b->n_items, chunk,
