@@ -1686,10 +1686,11 @@ void enqueue_level(const g500_msbfs* b, int K, int L, hipStream_t s) {
   }
 }
 
+// cap: the level buffers the batch's levels were queued for (the output
+// kernels skip themselves when the last of them is not empty)
 template <int K>
-void launch_output(const g500_msbfs* b, hipStream_t s) {
+void launch_output(const g500_msbfs* b, hipStream_t s, int cap) {
   const g500_graph* g = b->g;
-  const int cap = b->levels_cap();
   if (!b->level_arrays) {
     // point lookups read the level frontiers; no per-root arrays
   } else if (g->n % kT == 0) {
@@ -1726,20 +1727,19 @@ void launch_output(const g500_msbfs* b, hipStream_t s) {
   G500_CHECK_LAUNCH();
 }
 
-void enqueue_output(const g500_msbfs* b, int K, hipStream_t s) {
+void enqueue_output(const g500_msbfs* b, int K, hipStream_t s, int cap) {
   switch (K) {
-    case 1: launch_output<1>(b, s); break;
-    case 2: launch_output<2>(b, s); break;
-    case 3: launch_output<3>(b, s); break;
-    default: launch_output<4>(b, s); break;
+    case 1: launch_output<1>(b, s, cap); break;
+    case 2: launch_output<2>(b, s, cap); break;
+    case 3: launch_output<3>(b, s, cap); break;
+    default: launch_output<4>(b, s, cap); break;
   }
 }
 
 // The whole batch after the roots are on the device: reset, roots, the
-// guarded levels up to the last level buffer, outputs.
-void enqueue_batch(const g500_msbfs* b, int K, hipStream_t s) {
+// guarded levels filling the first cap level buffers, outputs.
+void enqueue_batch(const g500_msbfs* b, int K, int cap, hipStream_t s) {
   const g500_graph* g = b->g;
-  const int cap = b->levels_cap();
   const int64_t words = g->n_c * K;
   const unsigned igrid = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>(ceil_div(words, kT), (int64_t)num_cus(g->device) * 16));
@@ -1753,7 +1753,7 @@ void enqueue_batch(const g500_msbfs* b, int K, hipStream_t s) {
                      b->F[0]->get(), b->fb[0].get(), b->cnt.get(), b->fe.get(), b->act.get());
   G500_CHECK_LAUNCH();
   for (int L = 0; L + 1 < cap; ++L) enqueue_level(b, K, L, s);
-  enqueue_output(b, K, s);
+  enqueue_output(b, K, s, cap);
 }
 
 void add_level_buffer(g500_msbfs* b) {
@@ -1999,7 +1999,11 @@ void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots) {
   b->h_roots[0] = nroots;
   std::memcpy(b->h_roots + 1, roots, (size_t)nroots * sizeof(int64_t));
   hipStream_t s = b->stream;
-  if (b->tuning.use_graph && (!b->exec[K] || b->exec_levels[K] != b->levels_cap())) {
+  // levels queued: two more than the deepest batch so far needed (every
+  // guarded level step costs its empty launches), all buffers before that
+  const int ncap = b->depth_hint > 0 ? std::min(std::max(b->depth_hint, 3), b->levels_cap())
+                                     : b->levels_cap();
+  if (b->tuning.use_graph && (!b->exec[K] || b->exec_levels[K] != ncap)) {
     // captured once per K and level count; the stream is idle here (no batch
     // in flight), and the capture itself runs nothing
     {
@@ -2008,7 +2012,7 @@ void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots) {
       hipGraph_t graph = nullptr;
       G500_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_batch(b, K, s);
+        enqueue_batch(b, K, ncap, s);
       } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -2018,7 +2022,7 @@ void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots) {
       const hipError_t e = hipGraphInstantiate(&b->exec[K], graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) fail(G500_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-      b->exec_levels[K] = b->levels_cap();
+      b->exec_levels[K] = ncap;
     }
   }
   G500_CHECK_HIP(hipEventRecord(b->ev_begin, s));
@@ -2027,8 +2031,9 @@ void msbfs_launch(g500_msbfs* b, const int64_t* roots, int nroots) {
   if (b->tuning.use_graph) {
     G500_CHECK_HIP(hipGraphLaunch(b->exec[K], s));
   } else {
-    enqueue_batch(b, K, s);
+    enqueue_batch(b, K, ncap, s);
   }
+  b->run_levels = ncap;
   G500_CHECK_HIP(hipEventRecord(b->ev_end, s));
   b->nroots = nroots;
   b->K = K;
@@ -2058,7 +2063,7 @@ void msbfs_wait(g500_msbfs* b) {
   const size_t cbytes = (kMaxLevels + 2) * sizeof(unsigned long long);
   G500_CHECK_HIP(hipMemcpyAsync(b->h_cnt, b->cnt.get(), cbytes, hipMemcpyDeviceToHost, s));
   G500_CHECK_HIP(hipStreamSynchronize(s));
-  int cap = b->levels_cap();
+  int cap = b->run_levels;
   if (b->h_cnt[cap - 1] != 0) {
     // deeper than the level buffers: continue level by level, then redo the
     // outputs (the captured output kernels skipped themselves)
@@ -2075,7 +2080,7 @@ void msbfs_wait(g500_msbfs* b) {
              kMaxLevels - 1);
     }
     // the last buffer now holds the empty frontier, so cnt[cap - 1] == 0
-    enqueue_output(b, b->K, s);
+    enqueue_output(b, b->K, s, b->levels_cap());
     G500_CHECK_HIP(hipEventRecord(b->ev_end, s));
     G500_CHECK_HIP(hipEventSynchronize(b->ev_end));
     G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_begin, b->ev_end));
@@ -2096,6 +2101,7 @@ void msbfs_wait(g500_msbfs* b) {
       break;
     }
   st.nlevels = nlev;
+  b->depth_hint = std::max(b->depth_hint, nlev + 2);
   for (int l = 0; l < nlev; ++l) {
     st.discovered += (int64_t)b->h_cnt[l];
     if (l < 64) st.lane_discoveries[l] = (int64_t)b->h_cnt[l];

@@ -57,6 +57,8 @@ struct g500_msbfs {
   g500::DevBuf<uint64_t> split_mask;        // split level: the lanes that push (4 words)
   g500::ScanWorkspace scan_ws;
 
+  int run_levels = 0;   // level buffers the running batch's steps fill
+  int depth_hint = 0;   // levels of the deepest batch so far, plus two (0: none yet)
   // one instantiated graph per K (the level loop is captured once per K)
   hipGraphExec_t exec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int exec_levels[5] = {0, 0, 0, 0, 0};
