@@ -333,10 +333,52 @@ __device__ inline void ms_push_n(const MsLevel& a, const uint32_t (&w)[kPushIlp]
   }
 }
 
-// Top-down over new IDs [vb, ve), G lanes per vertex.
+// Pushes without a seen filter (the pulls mask F_{L+1} with seen): the
+// lanes of f missing from each target's word, ORed in; lanes only ever get
+// added, so a stale read can only cause an extra atomic.
+template <int K>
+__device__ inline void hyb_push_n(const MsLevel& a, const uint32_t (&w)[kPushIlp],
+                                  const bool (&ok)[kPushIlp], const uint64_t (&f)[K]) {
+  uint64_t x[kPushIlp][K];
+#pragma unroll
+  for (int u = 0; u < kPushIlp; ++u)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      x[u][k] = (uint64_t)__atomic_load_n((unsigned long long*)&a.fn[(int64_t)w[u] * K + k],
+                                          __ATOMIC_RELAXED);
+#pragma unroll
+  for (int u = 0; u < kPushIlp; ++u)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t m = ok[u] ? f[k] & ~x[u][k] : 0ull;
+      if (m) atomicOr((unsigned long long*)&a.fn[(int64_t)w[u] * K + k], (unsigned long long)m);
+    }
+}
+
+// Pushes from the frontier F_L over new IDs [vb, ve), G lanes per vertex: at
+// a top-down level every lane (filtered by seen: ms_push_n), at a split level
+// only the lanes in split_mask (hyb_push_n: the pulls queued after them mask
+// F_{L+1} with seen).  One kernel for both, so a level queues no push kernel
+// that only returns.
+template <int K>
+__device__ inline bool push_lanes(const MsLevel& a, bool split, uint64_t (&pm)[K]) {
+  bool anyp = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    pm[k] = split ? a.split_mask[k] : ~0ull;
+    anyp = anyp || pm[k] != 0;
+  }
+  return anyp;
+}
+
 template <int K, int G>
-__global__ __launch_bounds__(kT) void k_ms_td(MsLevel a, int64_t vb, int64_t ve) {
-  if (!ms_active(a) || !ms_top_down(a, a.L)) return;
+__global__ __launch_bounds__(kT) void k_ms_push(MsLevel a, int64_t vb, int64_t ve) {
+  if (!ms_active(a)) return;
+  const int mode = ms_mode(a, a.L);
+  if (mode != kModeTopDown && mode != kModeSplit) return;
+  const bool split = mode == kModeSplit;
+  uint64_t pm[K];
+  if (!push_lanes<K>(a, split, pm)) return;
   const int lig = threadIdx.x % G;
   const int64_t gid = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
   const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
@@ -344,23 +386,38 @@ __global__ __launch_bounds__(kT) void k_ms_td(MsLevel a, int64_t vb, int64_t ve)
     if (!in_bitmap(a.fb, v)) continue;
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      f[k] &= pm[k];
+      any = any || f[k] != 0;
+    }
+    if (!any) continue;
     const int64_t end = a.row_off[v + 1];
     for (int64_t e0 = a.row_off[v] + lig; e0 < end; e0 += (int64_t)G * kPushIlp) {
       uint32_t w[kPushIlp];
       bool ok[kPushIlp];
       push_targets(a.col, e0, G, end, w, ok);
-      ms_push_n<K>(a, w, ok, f);
+      if (split)
+        hyb_push_n<K>(a, w, ok, f);
+      else
+        ms_push_n<K>(a, w, ok, f);
     }
   }
 }
 
-// Top-down for heavy vertices: one wave per work item of `chunk` edges.
+// The same for heavy vertices: one wave per work item of `chunk` edges.
 template <int K>
-__global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
-                                                   const uint64_t* __restrict__ item_start,
-                                                   const uint32_t* __restrict__ item_vertex,
-                                                   int64_t n_items, int chunk) {
-  if (!ms_active(a) || !ms_top_down(a, a.L)) return;
+__global__ __launch_bounds__(kT) void k_ms_push_heavy(MsLevel a,
+                                                     const uint64_t* __restrict__ item_start,
+                                                     const uint32_t* __restrict__ item_vertex,
+                                                     int64_t n_items, int chunk) {
+  if (!ms_active(a)) return;
+  const int mode = ms_mode(a, a.L);
+  if (mode != kModeTopDown && mode != kModeSplit) return;
+  const bool split = mode == kModeSplit;
+  uint64_t pm[K];
+  if (!push_lanes<K>(a, split, pm)) return;
   const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
   const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
@@ -369,13 +426,23 @@ __global__ __launch_bounds__(kT) void k_ms_td_heavy(MsLevel a,
     if (!in_bitmap(a.fb, v)) continue;
     uint64_t f[K];
     load_lanes<K>(a.fl + v * K, f);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      f[k] &= pm[k];
+      any = any || f[k] != 0;
+    }
+    if (!any) continue;
     const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
     const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
     for (int64_t e0 = beg + lane; e0 < end; e0 += (int64_t)kWave * kPushIlp) {
       uint32_t w[kPushIlp];
       bool ok[kPushIlp];
       push_targets(a.col, e0, kWave, end, w, ok);
-      ms_push_n<K>(a, w, ok, f);
+      if (split)
+        hyb_push_n<K>(a, w, ok, f);
+      else
+        ms_push_n<K>(a, w, ok, f);
     }
   }
 }
@@ -682,27 +749,6 @@ __global__ __launch_bounds__(kT) void k_ms_hyb_pull_heavy(MsLevel a,
   }
 }
 
-// Pushes without a seen filter (the pulls mask F_{L+1} with seen): the
-// lanes of f missing from each target's word, ORed in; lanes only ever get
-// added, so a stale read can only cause an extra atomic.
-template <int K>
-__device__ inline void hyb_push_n(const MsLevel& a, const uint32_t (&w)[kPushIlp],
-                                  const bool (&ok)[kPushIlp], const uint64_t (&f)[K]) {
-  uint64_t x[kPushIlp][K];
-#pragma unroll
-  for (int u = 0; u < kPushIlp; ++u)
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      x[u][k] = (uint64_t)__atomic_load_n((unsigned long long*)&a.fn[(int64_t)w[u] * K + k],
-                                          __ATOMIC_RELAXED);
-#pragma unroll
-  for (int u = 0; u < kPushIlp; ++u)
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint64_t m = ok[u] ? f[k] & ~x[u][k] : 0ull;
-      if (m) atomicOr((unsigned long long*)&a.fn[(int64_t)w[u] * K + k], (unsigned long long)m);
-    }
-}
 
 // Push side of the hybrid: frontier vertices in [max(vb, hot), ve), G lanes each.
 template <int K, int G>
@@ -801,85 +847,6 @@ __global__ __launch_bounds__(kT) void k_ms_lane_mask(MsLevel a, int parts) {
   if (t < 4) a.split_mask[t] = mk[t];
 }
 
-// Push side of a split level, queued before the pulls: the frontier
-// vertices of [vb, ve), G lanes each, OR their pushing lanes into F_{L+1}
-// (cleared by k_ms_prep) along their rows; each vertex's pull then adds its
-// own lanes and masks the whole word with seen.
-template <int K, int G>
-__global__ __launch_bounds__(kT) void k_ms_split_push(MsLevel a, int64_t vb, int64_t ve) {
-  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
-  uint64_t pm[K];
-  bool anyp = false;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    pm[k] = a.split_mask[k];
-    anyp = anyp || pm[k] != 0;
-  }
-  if (!anyp) return;
-  const int lig = threadIdx.x % G;
-  const int64_t gid = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
-  const int64_t ngroups = (int64_t)gridDim.x * (kT / G);
-  for (int64_t v = vb + gid; v < ve; v += ngroups) {
-    if (!in_bitmap(a.fb, v)) continue;
-    uint64_t f[K];
-    load_lanes<K>(a.fl + v * K, f);
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      f[k] &= pm[k];
-      any = any || f[k] != 0;
-    }
-    if (!any) continue;
-    const int64_t end = a.row_off[v + 1];
-    for (int64_t e0 = a.row_off[v] + lig; e0 < end; e0 += (int64_t)G * kPushIlp) {
-      uint32_t w[kPushIlp];
-      bool ok[kPushIlp];
-      push_targets(a.col, e0, G, end, w, ok);
-      hyb_push_n<K>(a, w, ok, f);
-    }
-  }
-}
-
-// The same for heavy rows, one wave per work item of `chunk` edges.
-template <int K>
-__global__ __launch_bounds__(kT) void k_ms_split_push_heavy(MsLevel a,
-                                                           const uint64_t* __restrict__ item_start,
-                                                           const uint32_t* __restrict__ item_vertex,
-                                                           int64_t n_items, int chunk) {
-  if (!ms_active(a) || ms_mode(a, a.L) != kModeSplit) return;
-  uint64_t pm[K];
-  bool anyp = false;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    pm[k] = a.split_mask[k];
-    anyp = anyp || pm[k] != 0;
-  }
-  if (!anyp) return;
-  const int lane = lane_id();
-  const int64_t wid = ((int64_t)blockIdx.x * kT + threadIdx.x) / kWave;
-  const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
-  for (int64_t it = wid; it < n_items; it += nw) {
-    const int64_t v = item_vertex[it];
-    if (!in_bitmap(a.fb, v)) continue;
-    uint64_t f[K];
-    load_lanes<K>(a.fl + v * K, f);
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      f[k] &= pm[k];
-      any = any || f[k] != 0;
-    }
-    if (!any) continue;
-    const int64_t beg = a.row_off[v] + (int64_t)(it - item_start[v]) * chunk;
-    const int64_t end = min(a.row_off[v + 1], beg + (int64_t)chunk);
-    for (int64_t e0 = beg + lane; e0 < end; e0 += (int64_t)kWave * kPushIlp) {
-      uint32_t w[kPushIlp];
-      bool ok[kPushIlp];
-      push_targets(a.col, e0, kWave, end, w, ok);
-      hyb_push_n<K>(a, w, ok, f);
-    }
-  }
-}
 
 // After a streaming level: F_{L+1} = gathered lanes & ~seen, seen |= F_{L+1},
 // frontier bitmap, counters.
@@ -1517,18 +1484,10 @@ MsLevel level_args(const g500_msbfs* b, int L) {
 }
 
 template <int K, int G>
-void launch_td(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve, hipStream_t s) {
+void launch_push(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve, hipStream_t s) {
   if (ve <= vb) return;
-  hipLaunchKernelGGL((k_ms_td<K, G>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s, a, vb, ve);
-  G500_CHECK_LAUNCH();
-}
-
-template <int K, int G>
-void launch_split_push(const g500_msbfs* b, const MsLevel& a, int64_t vb, int64_t ve,
-                       hipStream_t s) {
-  if (ve <= vb) return;
-  hipLaunchKernelGGL((k_ms_split_push<K, G>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s, a,
-                     vb, ve);
+  hipLaunchKernelGGL((k_ms_push<K, G>), dim3(sweep_grid(b, ve - vb, G)), dim3(kT), 0, s, a, vb,
+                     ve);
   G500_CHECK_LAUNCH();
 }
 
@@ -1577,22 +1536,30 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
       1, std::min<int64_t>(ceil_div(n_c * K, kT), (int64_t)num_cus(b->g->device) * 8));
   hipLaunchKernelGGL((k_ms_prep<K>), dim3(pgrid), dim3(kT), 0, s, a, bw);
   G500_CHECK_LAUNCH();
+  // a split level's pushing lanes (split levels need split_bp > 0 and come
+  // after a top-down level)
+  if (a.split_thr > 0 && L > 0) {
+    hipLaunchKernelGGL((k_ms_lane_mask<K>), dim3(1), dim3(kT), 0, s, a, kLaneParts);
+    G500_CHECK_LAUNCH();
+  }
+  // pushes: every lane at a top-down level, the pushing lanes at a split
+  // level (before the pulls, which take the pushed lanes into F_{L+1})
   const unsigned hgrid = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>(ceil_div(b->n_items, kWavesPerBlock),
                            (int64_t)num_cus(b->g->device) * b->tuning.blocks_per_cu));
   if (b->n_items > 0) {
-    hipLaunchKernelGGL((k_ms_td_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
+    hipLaunchKernelGGL((k_ms_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
                        b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
     G500_CHECK_LAUNCH();
   }
   switch (b->tuning.medium_group) {
-    case 2: launch_td<K, 2>(b, a, h, m, s); break;
-    case 4: launch_td<K, 4>(b, a, h, m, s); break;
-    case 8: launch_td<K, 8>(b, a, h, m, s); break;
-    case 16: launch_td<K, 16>(b, a, h, m, s); break;
-    default: launch_td<K, 32>(b, a, h, m, s); break;
+    case 2: launch_push<K, 2>(b, a, h, m, s); break;
+    case 4: launch_push<K, 4>(b, a, h, m, s); break;
+    case 8: launch_push<K, 8>(b, a, h, m, s); break;
+    case 16: launch_push<K, 16>(b, a, h, m, s); break;
+    default: launch_push<K, 32>(b, a, h, m, s); break;
   }
-  launch_td<K, 1>(b, a, m, n_c, s);
+  launch_push<K, 1>(b, a, m, n_c, s);
   hipLaunchKernelGGL((k_ms_td_count<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a);
   G500_CHECK_LAUNCH();
   // Every kernel below is queued for every level and returns at once unless
@@ -1638,25 +1605,6 @@ void level_kernels(const g500_msbfs* b, int L, hipStream_t s) {
     hipLaunchKernelGGL((k_ms_commit<K>), dim3(sweep_grid(b, n_c, 1)), dim3(kT), 0, s, a,
                        (int)kModeStream);
     G500_CHECK_LAUNCH();
-  }
-  // split level: the pushing lanes, then their pushes, before the pulls
-  // (which take the pushed lanes into F_{L+1} with their own)
-  if (a.split_thr > 0 && L > 0) {
-    hipLaunchKernelGGL((k_ms_lane_mask<K>), dim3(1), dim3(kT), 0, s, a, kLaneParts);
-    G500_CHECK_LAUNCH();
-    if (b->n_items > 0) {
-      hipLaunchKernelGGL((k_ms_split_push_heavy<K>), dim3(hgrid), dim3(kT), 0, s, a,
-                         b->item_start.get(), b->item_vertex.get(), b->n_items, chunk);
-      G500_CHECK_LAUNCH();
-    }
-    switch (b->tuning.medium_group) {
-      case 2: launch_split_push<K, 2>(b, a, h, m, s); break;
-      case 4: launch_split_push<K, 4>(b, a, h, m, s); break;
-      case 8: launch_split_push<K, 8>(b, a, h, m, s); break;
-      case 16: launch_split_push<K, 16>(b, a, h, m, s); break;
-      default: launch_split_push<K, 32>(b, a, h, m, s); break;
-    }
-    launch_split_push<K, 1>(b, a, m, n_c, s);
   }
   if (b->n_items > 0) {
     hipLaunchKernelGGL((k_ms_bu_heavy<K, 4>), dim3(hgrid), dim3(kT), 0, s, a,
