@@ -100,11 +100,11 @@ struct g500_bfs {
   hipStream_t gstream = nullptr;
   g500::DevBuf<uint64_t> vis_snap;
   // ev_snap follows the last conversion's count pass (the gather waits on
-  // it), ev_gather the last gather queued (the engine stream waits on it
-  // before the fixup and the next run).  Events, not value waits: a value
+  // it); the engine stream waits on ev_g1, behind the last gather queued,
+  // before the fixup and the next run.  Events, not value waits: a value
   // wait polls from a kernel, which never ends where the kernels of a device
   // run one at a time (as under a counter profiler)
-  hipEvent_t ev_snap = nullptr, ev_gather = nullptr;
+  hipEvent_t ev_snap = nullptr;
   g500::DevBuf<uint64_t> lpart;   // per-block totals of a level's kernels
   g500::DevBuf<uint32_t> qcnt;    // per-wave counts of the bitmap -> queue conversion
   g500::DevBuf<uint64_t> qedges;  // its per-block degree sums
@@ -118,7 +118,8 @@ struct g500_bfs {
   std::vector<hipEvent_t> level_events;  // 2 per launched step when timing is on
   bool timing = false;
   // per-kernel timing (always on): the gather's events on the stream it runs
-  // on, and a pair around each k_bu_sweep launch of a run
+  // on (ev_g1 also orders the engine stream after the gather), and a pair
+  // around each k_bu_sweep launch of a run
   hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;
   std::vector<hipEvent_t> sweep_events;  // 2 per launched sweep
   // counting runs: per-block row-scan bytes of the bottom-up kernels, one
