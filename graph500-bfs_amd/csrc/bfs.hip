@@ -1413,7 +1413,8 @@ void bfs_init(g500_bfs* b, int edgefactor, double alpha, double beta) {
                                hipHostMallocMapped | hipHostMallocCoherent));
   std::memset((void*)b->h_report, 0, kReportSlots * sizeof(HostReport));
   G500_CHECK_HIP(hipHostGetDevicePointer((void**)&b->d_report, b->h_report, 0));
-  G500_CHECK_HIP(hipEventCreate(&b->ev_begin));
+  // ev_begin only times the run; ev_end also tells the host the run is done
+  G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_begin, hipEventDisableSystemFence));
   G500_CHECK_HIP(hipEventCreate(&b->ev_end));
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_snap, hipEventDisableTiming));
   G500_CHECK_HIP(hipEventCreateWithFlags(&b->ev_g0, hipEventDisableSystemFence));
