@@ -18,7 +18,8 @@ device time per BFS in the timed runs, from HIP events on the stream it runs
 on): its algorithmic bytes per launch -- counted on the device in an untimed
 counting pass over the same roots -- over its mean launch time.  A second
 figure does the same for the whole BFS, and the top-down yardstick
-4 m_c + 8 (n_c+1) + 12 N of the north star is reported under its own name.
+4 m_c + 8 (n_c+1) + 12 N (every component edge read once, as a top-down
+BFS would) is reported under its own name.
 
 Secondary measurements on the same line: "scale24" (the SCALE-24 drop-in
 case, 64 validated roots, with the device time between levels), "batched"
@@ -439,8 +440,8 @@ def main():
     traffic = profiled_traffic(args, dom)
     bfs_gbs = float(np.mean(alg_runs)) / mean_gpu_s / 1e9
 
-    # the north star's top-down yardstick for one root's BFS (same component
-    # for every root)
+    # the top-down yardstick for one root's BFS (same component for every
+    # root)
     r0 = roots[0]
     bfs.run(r0, args.edgefactor, args.alpha, args.beta)
     if args.levels:

@@ -1,8 +1,8 @@
 """Extracts a known-answer fixture for the Graph500 result block from a log of
 the reference driver (mpi/main.cc:632-702, mpi/logfile.h:54-133).
 
-Input: the stdout of the reference `runnable 16` at one rank (the project
-survey kept one as ref_s16.log).  Output: per-root BFS times (max precision
+Input: the stdout of the reference `runnable 16` at one rank (a log of one
+such run is ref_s16.log).  Output: per-root BFS times (max precision
 line), validate times, traversed edges, and the result block values.
 usage: python make_ref_result_block.py ref_s16.log > ref_s16_result_block.json
 """
