@@ -1895,13 +1895,15 @@ void msbfs_init(g500_msbfs* b, int max_roots, bool level_arrays) {
                        b->stream, g->row_off.get(), g->n_c, b->stream_tile_v0.get());
     G500_CHECK_LAUNCH();
   }
+  // defaults: what the autotuner picked at SCALE 26 for 64 and 128 roots
+  // (and near what it picks for 256)
   g500_msbfs_tuning t{};
-  t.heavy_degree = 2048;
-  t.medium_degree = 64;
-  t.medium_group = 8;
-  t.blocks_per_cu = 8;
+  t.heavy_degree = 8192;
+  t.medium_degree = 256;
+  t.medium_group = 16;
+  t.blocks_per_cu = 16;
   t.use_graph = 1;
-  t.heavy_chunk = 4096;
+  t.heavy_chunk = 8192;
   t.td_alpha = 4.0;
   t.stream_frac = 0.0;  // dense levels off until measured faster (tests force them)
   t.hot_bytes = 2 << 20;
