@@ -1288,18 +1288,30 @@ __global__ __launch_bounds__(kT) void k_pred_gather(const uint32_t* __restrict__
 // entry is always -1: the engine fills pred once and the gather walks only
 // core, the (original ID, new ID) pairs of the vertices with edges in
 // ascending original-ID order.  One pair per thread: per vertex one
-// random 8-byte read (its entry), the rest streams.
+// random 8-byte read (its entry), the rest streams.  Started at the switch
+// back to top-down (snap_level > 0: the level of that conversion), it writes
+// only the vertices inside the bitmap snapshot -- this run's entries below
+// that level, which nothing writes again -- and leaves the rest to
+// k_pred_fixup, so the two write disjoint entries and need no order between
+// them.  An entry the top-down levels write while the gather reads it is
+// outside the snapshot whether it is read before (another epoch) or after
+// (its level).
 __global__ __launch_bounds__(kT) void k_pred_gather_core(const uint2* __restrict__ core,
                                                         int64_t n_core, uint32_t epoch,
                                                         const int64_t* __restrict__ pred_new,
                                                         int64_t root,
                                                         int64_t* __restrict__ pred,
+                                                        uint32_t snap_level,
                                                         const LevelSync* ls, uint32_t snap_code) {
   if (!snapshot_is(ls, snap_code)) return;
   const XcdSpan xs = xcd_span(n_core);
   for (int64_t i = xs.lo + xs.rank * kT + threadIdx.x; i < xs.hi; i += xs.nblk * kT) {
     const uint2 ov = get_pair(core + i);
-    int64_t e = run_entry(pred_new[ov.y], epoch);
+    const int64_t raw = pred_new[ov.y];
+    if (snap_level != 0 && (run_entry(raw, epoch) == -1 ||
+                            (uint32_t)((uint64_t)raw >> 48) >= snap_level))
+      continue;
+    int64_t e = run_entry(raw, epoch);
     if ((int64_t)ov.x == root) e = root;
     __builtin_nontemporal_store(e, pred + ov.x);
   }
@@ -1319,26 +1331,32 @@ __global__ __launch_bounds__(kT) void k_core_list(const uint32_t* __restrict__ f
   if (o < n && flag[o]) core[off[o]] = make_uint2((uint32_t)o, o2n[o]);
 }
 
-// pred entries of the vertices reached after the snapshot the early gather
-// used (cur & ~snap): one thread per bitmap word, entries scattered by
-// original ID.  Queued on a guess that the run ends (done_code != 0), it
-// returns at once unless the loop state says so.
+// pred entries of the vertices outside the snapshot the early gather used:
+// one thread per bitmap word, entries scattered by original ID.  With
+// `outside` (the gather skipped those vertices) every vertex outside the
+// snapshot gets its entry if the run reached it and -1 if not; otherwise
+// (the gather wrote every vertex, and this runs after it) the ones reached
+// after the snapshot (cur & ~snap) get their entries.  Queued on a guess
+// that the run ends (done_code != 0), it returns at once unless the loop
+// state says so.
 __global__ __launch_bounds__(kT) void k_pred_fixup(const uint64_t* __restrict__ cur,
                                                   const uint64_t* __restrict__ snap,
                                                   int64_t words,
                                                   const uint32_t* __restrict__ n2o,
                                                   const int64_t* __restrict__ pred_new,
-                                                  int64_t* __restrict__ pred,
+                                                  int64_t* __restrict__ pred, int outside,
                                                   const LevelSync* ls, uint32_t done_code) {
   if (done_code != 0 && !step_is_next(ls, done_code)) return;
   const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (w >= words) return;
-  uint64_t bits = cur[w] & ~snap[w];
+  const uint64_t c = cur[w];
+  uint64_t bits = outside ? ~snap[w] : c & ~snap[w];
   while (bits) {
     const int b = __ffsll((long long)bits) - 1;
     bits &= bits - 1;
     const uint32_t v = (uint32_t)(w * 64 + b);
-    pred[n2o[v]] = (int64_t)((uint64_t)pred_new[v] & ~kEpochMask);
+    pred[n2o[v]] = ((c >> b) & 1ull) ? (int64_t)((uint64_t)pred_new[v] & ~kEpochMask)
+                                     : (int64_t)-1;
   }
 }
 
@@ -1512,7 +1530,8 @@ void wait_report(g500_bfs* b, unsigned long long seq, unsigned long long out[4])
 // the last top-down levels measured slower: the random reads need every
 // wave the chip holds).  snap_code != 0: queued before the conversion of
 // that code is known to run (the kernel checks).
-void launch_gather(g500_bfs* b, int64_t root, hipStream_t st, uint32_t snap_code) {
+void launch_gather(g500_bfs* b, int64_t root, hipStream_t st, uint32_t snap_code,
+                   uint32_t snap_level) {
   g500_graph* g = b->g;
   G500_CHECK_HIP(hipEventRecord(b->ev_g0, st));
   // grids are a multiple of kXcds (xcd_span)
@@ -1522,7 +1541,7 @@ void launch_gather(g500_bfs* b, int64_t root, hipStream_t st, uint32_t snap_code
   if (b->n_core > 0)
     hipLaunchKernelGGL(k_pred_gather_core, dim3(grid(b->n_core)), dim3(kT), 0, st,
                        b->core.get(), b->n_core, b->epoch, b->pred_new.get(), root,
-                       b->pred.get(), b->lsync.get(), snap_code);
+                       b->pred.get(), snap_level, b->lsync.get(), snap_code);
   else
     hipLaunchKernelGGL(k_pred_gather, dim3(grid(g->n)),
                        dim3(kT), 0, st, g->orig2new.get(), b->epoch, b->pred_new.get(), g->n,
@@ -1792,20 +1811,30 @@ struct EndGuess {
 // The engine stream waits until every gather queued so far has ended.
 void wait_gathers(g500_bfs* b) { G500_CHECK_HIP(hipStreamWaitEvent(b->stream, b->ev_g1, 0)); }
 
+// Without CoreBFS the early gather writes only the vertices inside the
+// bitmap snapshot and the fixup all the others, so the fixup need not wait
+// for the gather (a handoff between streams costs ~25 us); the host waits
+// for both streams at the end of the run.  With CoreBFS the gather writes
+// every entry (forest parents included) and the fixup follows it.
+bool disjoint_fixup(const g500_bfs* b) { return b->n_core > 0; }
+
 // The gather on gstream behind the snapshot of the conversion queued last
-// (ev_snap; snap_code: its code when it is queued on a guess); ev_g1, which
-// launch_gather records behind it, is what the engine stream waits on.
-void queue_gather(g500_bfs* b, int64_t root, uint32_t snap_code) {
+// (ev_snap; conv_code: that conversion's code; snap_code: the same when it
+// is queued on a guess, else 0); ev_g1, which launch_gather records behind
+// it, is what the engine stream waits on.
+void queue_gather(g500_bfs* b, int64_t root, uint32_t snap_code, uint32_t conv_code) {
   G500_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_snap, 0));
-  launch_gather(b, root, b->gstream, snap_code);
+  launch_gather(b, root, b->gstream, snap_code,
+                disjoint_fixup(b) ? (uint32_t)step_level(conv_code) : 0u);
 }
 
 void queue_fixup(g500_bfs* b, const uint64_t* cur, uint32_t done_code) {
   hipStream_t s = b->stream;
-  wait_gathers(b);
+  const bool outside = disjoint_fixup(b);
+  if (!outside) wait_gathers(b);
   hipLaunchKernelGGL(k_pred_fixup, dim3((unsigned)ceil_div(b->words, kT)), dim3(kT), 0, s, cur,
                      b->vis_snap.get(), b->words, b->g->new2orig.get(), b->pred_new.get(),
-                     b->pred.get(), b->lsync.get(), done_code);
+                     b->pred.get(), outside ? 1 : 0, b->lsync.get(), done_code);
   G500_CHECK_LAUNCH();
 }
 
@@ -1871,7 +1900,7 @@ void speculate(g500_bfs* b, std::deque<Step>& pend, const HostLoop& hl, size_t d
     pend.push_back(launch_step(b, next, last.after, nlev_ev, nsweep_ev));
     if (step_mode(next) == kModeConv && !eg->gather_confirmed) {
       // ev_snap was just recorded behind this conversion's snapshot
-      queue_gather(b, eg->root, next);
+      queue_gather(b, eg->root, next, next);
       eg->conv_seq = pend.back().seq;
     }
   }
@@ -2076,7 +2105,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
       // the vertices the last levels reach are patched in at the end
       // (k_pred_fixup).  A conversion queued on a guess has its gather
       // queued behind it already.
-      if (eg.conv_seq != pend.front().seq) queue_gather(b, root, 0);
+      if (eg.conv_seq != pend.front().seq) queue_gather(b, root, 0, next);
       early_gather = true;
       eg.gather_confirmed = true;
     }
@@ -2086,19 +2115,27 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   b->vis_alt.p = cur.vis_alt;
   b->q.p = cur.q;
   b->nq.p = cur.nq;
+  // the gather: per vertex with edges its original ID, new ID, entry read and
+  // written; with CoreBFS every original ID (new ID and entry written) and
+  // every vertex with edges' entry
+  int64_t gather_bytes = b->n_core > 0 ? 24 * b->n_core : 12 * g->n + 8 * n_c;
+  const bool disjoint = early_gather && disjoint_fixup(b);
   if (early_gather) {
     // unless the guessed fixup followed the step that ended the run
     if (!(eg.fixup_seq == ran.back().seq && eg.fixup_code == final_code))
       queue_fixup(b, b->vis.get(), 0);
     // both bitmaps; per vertex its new -> original ID, entry read and written
     alg += 16 * words + 20 * after_snap;
+    if (disjoint) {
+      // the fixup also writes -1 for the vertices never reached, which the
+      // gather leaves out with those reached after the snapshot
+      const int64_t unreached = std::max<int64_t>(0, n_c - hl.visited);
+      alg += 12 * unreached;
+      gather_bytes -= 8 * (after_snap + unreached);
+    }
   } else {
-    launch_gather(b, root, s, 0);
+    launch_gather(b, root, s, 0, 0u);
   }
-  // the gather: per vertex with edges its original ID, new ID, entry read and
-  // written; with CoreBFS every original ID (new ID and entry written) and
-  // every vertex with edges' entry
-  const int64_t gather_bytes = b->n_core > 0 ? 24 * b->n_core : 12 * g->n + 8 * n_c;
   alg += gather_bytes;
   if (b->n_core > 0) {
     // the gather skips vertices without edges: a root among them gets its
@@ -2124,6 +2161,13 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   G500_CHECK_HIP(hipEventSynchronize(b->ev_end));
   float ms = 0;
   G500_CHECK_HIP(hipEventElapsedTime(&ms, b->ev_begin, b->ev_end));
+  if (disjoint) {
+    // the gather may end after the engine stream's last kernel
+    G500_CHECK_HIP(hipEventSynchronize(b->ev_g1));
+    float gms = 0;
+    G500_CHECK_HIP(hipEventElapsedTime(&gms, b->ev_begin, b->ev_g1));
+    ms = std::max(ms, gms);
+  }
   b->stats.gpu_ms = ms;
   b->stats.nlevels = (int)b->trace.size();
   b->stats.mispredicted = mispredicted;
