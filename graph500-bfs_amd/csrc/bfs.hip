@@ -1550,6 +1550,7 @@ void launch_gather(g500_bfs* b, int64_t root, hipStream_t st, uint32_t snap_code
                        b->lsync.get(), snap_code);
   G500_CHECK_LAUNCH();
   G500_CHECK_HIP(hipEventRecord(b->ev_g1, st));
+  if (st != b->stream) b->gather_pending = true;
 }
 
 // The buffers a step starts from: the visited bitmap, the other bitmap (the
@@ -1809,7 +1810,10 @@ struct EndGuess {
 };
 
 // The engine stream waits until every gather queued so far has ended.
-void wait_gathers(g500_bfs* b) { G500_CHECK_HIP(hipStreamWaitEvent(b->stream, b->ev_g1, 0)); }
+void wait_gathers(g500_bfs* b) {
+  G500_CHECK_HIP(hipStreamWaitEvent(b->stream, b->ev_g1, 0));
+  b->gather_pending = false;
+}
 
 // Without CoreBFS the early gather writes only the vertices inside the
 // bitmap snapshot and the fixup all the others, so the fixup need not wait
@@ -1954,12 +1958,13 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   // this run's epoch; before the epochs wrap, every entry is set to the
   // epoch no run uses, so no entry of an earlier run can match
   if (++b->epoch >= kEpochNone) {
-    wait_gathers(b);
+    if (b->gather_pending) wait_gathers(b);
     G500_CHECK_HIP(hipMemsetAsync(b->pred_new.get(), 0xFF, b->pred_new.n * sizeof(int64_t), s));
     b->epoch = 0;
   }
   // a gather left running by a run that failed must not overlap this one
-  wait_gathers(b);
+  // (after a run that ended normally the host or the fixup waited for it)
+  if (b->gather_pending) wait_gathers(b);
   if (b->n_core > 0 && b->pred_fill_needed) {
     // entries of vertices without edges are -1 and stay so; the fill is
     // redone only after the buffer was overwritten (validation of a host pred)
@@ -2164,6 +2169,7 @@ void bfs_run(g500_bfs* b, int64_t root, int edgefactor, double alpha,
   if (disjoint) {
     // the gather may end after the engine stream's last kernel
     G500_CHECK_HIP(hipEventSynchronize(b->ev_g1));
+    b->gather_pending = false;
     float gms = 0;
     G500_CHECK_HIP(hipEventElapsedTime(&gms, b->ev_begin, b->ev_g1));
     ms = std::max(ms, gms);

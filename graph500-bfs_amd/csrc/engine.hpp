@@ -121,6 +121,7 @@ struct g500_bfs {
   // on (ev_g1 also orders the engine stream after the gather), and a pair
   // around each k_bu_sweep launch of a run
   hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;
+  bool gather_pending = false;  // a gather on gstream neither stream nor host has waited for
   std::vector<hipEvent_t> sweep_events;  // 2 per launched sweep
   // counting runs: per-block row-scan bytes of the bottom-up kernels, one
   // slot per block (sweep, long-row and list kernels in that order)
