@@ -238,6 +238,35 @@ def test_validator_agrees_on_planted_mistakes(G):
     assert og.validate(r2, bfs.pred())[0]
 
 
+def test_runs_alternating_between_components(G):
+    """One engine runs roots of a small component and of the giant one in
+    turn.  The giant-component runs switch back to top-down, so their pred
+    gather starts early and covers only the vertices inside its snapshot;
+    the fixup must then reset the entries the small-component run wrote
+    (their vertices are outside the snapshot and unreached), and the next
+    small-component run must clear what the giant run wrote."""
+    print("\n[components] scale=16", flush=True)
+    g = G.Graph.generate(16).construct()
+    o0, o1 = O.generate_edges(16)
+    og = O.Graph(o0, o1)
+    big = int(g.find_roots(1)[0])
+    lev_big = og.bfs(big)
+    small = [v for v in range(og.info.n) if og.has_edge(v) and lev_big[v] < 0]
+    assert small, "the graph has vertices with edges outside the giant component"
+    bfs = G.Bfs(g)
+    for r in (small[0], big, small[-1], big, small[0]):
+        bfs.run(r)
+        assert np.array_equal(bfs.levels(), _oracle_levels(og, r)), r
+        pred = bfs.pred()
+        ok, _, errs = og.validate(r, pred)
+        assert ok, (r, errs)
+        if r == big:
+            # a top-down level after a bottom-up one: the early gather ran
+            dirs = [t["top_down"] for t in bfs.stats()[1]]
+            assert 0 in dirs and 1 in dirs[dirs.index(0):], dirs
+            assert all(pred[v] == -1 for v in small), r
+
+
 def test_roots_without_edges(G):
     """Roots that have no edges reach only themselves; the next run, from
     any root, leaves no trace of them in the parent array."""
